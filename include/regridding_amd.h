@@ -1,0 +1,168 @@
+/*
+ * regridding_amd.h -- C ABI of the MI355X conservative regridding library
+ * (librg_hip.so, built for gfx950).
+ *
+ * Plain pointers and sizes only.  Unless stated otherwise every pointer
+ * argument is DEVICE memory on the current device, every function returns an
+ * rg_status (0 = success) and, on failure, leaves a message retrievable with
+ * rg_last_error() on the calling thread.  `stream` is a hipStream_t (NULL
+ * selects the null stream); work is enqueued on it in order.  Cell arrays
+ * are flat, row-major over the two resampled axes, exactly like the flat
+ * cell indices of the reference (_weights_conservative_2d/_arrays.py:70-91).
+ *
+ * Each entry point names the reference function it replaces
+ * (paths relative to the reference repository's regridding/ package).
+ */
+#ifndef REGRIDDING_AMD_H
+#define REGRIDDING_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum rg_status {
+    RG_OK = 0,
+    RG_ERR_ARGUMENT = 1,  /* invalid sizes / null pointers (the reference raises
+                             ValueError before its compiled call) */
+    RG_ERR_HIP = 2,       /* a HIP runtime call failed */
+    RG_ERR_MEMORY = 3,    /* device allocation failed */
+    RG_ERR_GEOMETRY = 4,  /* the sweep did not terminate (exactly degenerate
+                             corner crossings, where the reference loops) */
+    RG_ERR_INJECTED = 5   /* failure forced through rg_inject_failure */
+};
+
+/* Opaque device-resident weights of one 2-D element. */
+typedef struct rg_weights* rg_weights_t;
+
+/* Tuning of the build.  Zero fields select the defaults. */
+typedef struct rg_tuning {
+    int32_t edges_per_lane; /* sweep edges walked by one lane (default 16) */
+    int32_t block;          /* threads per block of the walk kernels (128) */
+    int32_t flags;          /* RG_TUNE_TIMINGS: record per-phase times */
+    int32_t reserved;
+} rg_tuning;
+
+#define RG_TUNE_TIMINGS 1
+
+/* Counters and (optionally) phase times of a build. */
+typedef struct rg_build_stats {
+    int64_t n_triples;      /* uncoalesced (in, out, w) triples emitted */
+    int64_t n_chunks;       /* lanes of the walk */
+    int64_t n_fixups;       /* chunks re-walked by the consistency pass */
+    int64_t n_subsegments;  /* inside sub-segments (the FP64 work unit) */
+    int64_t n_search_anomalies; /* row starts inside the boundary polygon
+                                   that the cell search did not place */
+    double ms_prepare, ms_start, ms_count, ms_fixup, ms_emit, ms_reduce,
+        ms_transpose, ms_total;
+} rg_build_stats;
+
+/* Library / device. */
+const char* rg_last_error(void);
+int rg_version(void);
+int rg_init(int device);
+int rg_device_count(int* count);
+int rg_mem_info(size_t* free_bytes, size_t* total_bytes);
+void rg_trim(void); /* release cached device blocks */
+
+/* Device memory plumbing for hosts without their own allocator (ctypes). */
+int rg_malloc(void** ptr, size_t bytes);
+int rg_free(void* ptr);
+int rg_memcpy_htod(void* dst, const void* src, size_t bytes, void* stream);
+int rg_memcpy_dtoh(void* dst, const void* src, size_t bytes, void* stream);
+int rg_memset(void* dst, int value, size_t bytes, void* stream);
+int rg_stream_create(void** stream);
+int rg_stream_destroy(void* stream);
+int rg_stream_sync(void* stream);
+int rg_device_sync(void);
+
+/* Replaces _weights_conservative_2d/_grids.py:50 grid_volume: signed
+ * shoelace area of every cell of an (nx, ny) vertex grid, accumulated in the
+ * reference's order.  area has (nx-1)*(ny-1) entries. */
+int rg_cell_area_2d(const double* x, const double* y, int64_t nx, int64_t ny,
+                    double* area, void* stream);
+
+/* Replaces _weights_conservative_2d/_weights_conservative_2d.py:80
+ * weights_conservative_2d followed by _weights_arrays.py:44 _coalesce for one
+ * element: builds the coalesced sparse weights between an input grid
+ * (nxi, nyi) and an (already perturbed) output grid (nxo, nyo).  wi is the
+ * optional (nxi-1)*(nyi-1) input weighting or NULL.  tuning may be NULL.
+ * The returned handle owns its device memory; release with
+ * rg_weights_free. */
+int rg_build_conservative_2d(const double* xi, const double* yi, int64_t nxi,
+                             int64_t nyi, const double* xo, const double* yo,
+                             int64_t nxo, int64_t nyo, const double* wi,
+                             const rg_tuning* tuning, rg_weights_t* out,
+                             void* stream);
+
+/* Sizes and build counters of a handle. */
+int rg_weights_info(rg_weights_t w, int64_t* n_in, int64_t* n_out,
+                    int64_t* nnz);
+int rg_weights_stats(rg_weights_t w, rg_build_stats* stats);
+
+/* Replaces the output arrays of _weights_arrays.py:68-73: writes the nnz
+ * coalesced triples sorted by (input, output) into device arrays. */
+int rg_weights_export(rg_weights_t w, int64_t* indices_input,
+                      int64_t* indices_output, double* values, void* stream);
+
+/* Replaces the TypedList marshalling of _regrid_from_weights.py:139-141:
+ * adopts nnz (input, output, weight) triples in ANY order (device arrays),
+ * e.g. unpickled or transposed weights.  Applying the handle reproduces the
+ * reference's sequential scatter in storage order bit for bit. */
+int rg_weights_import(const int64_t* indices_input,
+                      const int64_t* indices_output, const double* values,
+                      int64_t nnz, int64_t n_in, int64_t n_out,
+                      rg_weights_t* out, void* stream);
+
+int rg_weights_free(rg_weights_t w);
+
+/* Replaces _regrid/_regrid_from_weights.py:166 _regrid_from_weights for
+ * `batch` elements that share one weight matrix: values_input is
+ * (batch, n_in), values_output (batch, n_out), both contiguous.
+ * transpose=0 applies the weights (out[o] = sum_i w[i,o] in[i]);
+ * transpose=1 applies the transposed weights, which is what the reference
+ * computes for regrid_from_weights(*transpose_weights(weights), ...)
+ * (_weights_transposed.py:13-50).  Each output is accumulated sequentially
+ * in the reference's storage order, so the FP64 result is bit-identical. */
+int rg_apply_f64(rg_weights_t w, int transpose, const double* values_input,
+                 double* values_output, int64_t batch, void* stream);
+
+/* Single-precision variant: FP32 values, FP32 weights (rounded from the
+ * FP64 weights), FP32 accumulation in the same order.  No reference
+ * counterpart (the reference always accumulates in float64). */
+int rg_apply_f32(rg_weights_t w, int transpose, const float* values_input,
+                 float* values_output, int64_t batch, void* stream);
+
+/* Replaces _weights_conservative_2d/_grids.py:356 index_of_point_secant
+ * (with its index_of_point_brute fallback, :223), vectorised over n query
+ * points: cell_ij receives 2*n int64 values (i, j) per point, or
+ * (INT64_MAX, INT64_MAX) for points outside the grid. */
+int rg_find_cells_2d(const double* x, const double* y, int64_t nx, int64_t ny,
+                     const double* px, const double* py, int64_t n,
+                     int64_t* cell_ij, void* stream);
+
+#define RG_FIND_BRUTE 0
+#define RG_FIND_SEARCHSORTED 1
+
+/* Replaces _find_indices/_find_indices_brute.py:25 and
+ * _find_indices_searchsorted.py:25 (_find_indices_*_1d): for each of `batch`
+ * elements, the cell of the m-vertex input grid containing each of the n
+ * output points, or fill_value.  x_input is (batch, m), x_output (batch, n),
+ * result (batch, n). */
+int rg_find_indices_1d(const double* x_input, int64_t m,
+                       const double* x_output, int64_t n, int64_t batch,
+                       int method, int64_t fill_value, int64_t* result,
+                       void* stream);
+
+/* Test hook (no reference counterpart): make the next call that reaches the
+ * named site ("build", "apply", "alloc") fail with RG_ERR_INJECTED.  NULL
+ * clears pending injections. */
+int rg_inject_failure(const char* site);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* REGRIDDING_AMD_H */

@@ -1,0 +1,233 @@
+"""
+ctypes front end of the CPU oracle (``oracle/librg_oracle.so``).
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()``
+and the ``cpu_baseline`` leg of ``bench.py`` -- never by ``regridding_amd``.
+The C source (``rg_oracle.c``) restates the reference's conservative 2-D path
+function by function; this module only marshals numpy arrays.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "librg_oracle.so")
+
+_i64 = ctypes.c_int64
+_dbl = ctypes.c_double
+_pd = ctypes.POINTER(ctypes.c_double)
+_pi = ctypes.POINTER(ctypes.c_int64)
+
+_lib = None
+
+
+def build() -> None:
+    """Compile the oracle with its Makefile (gcc, IEEE exact order)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.rgo_bbox_intersect.restype = ctypes.c_int
+        L.rgo_bbox_intersect.argtypes = [_dbl] * 8
+        L.rgo_seg_params.restype = None
+        L.rgo_seg_params.argtypes = [_pd, _pd]
+        L.rgo_seg_hit.restype = ctypes.c_int
+        L.rgo_seg_hit.argtypes = [_dbl, _dbl]
+        L.rgo_point_in_polygon.restype = ctypes.c_int
+        L.rgo_point_in_polygon.argtypes = [_dbl, _dbl, _pd, _pd, _i64]
+        L.rgo_grid_volume.restype = None
+        L.rgo_grid_volume.argtypes = [_pd, _pd, _i64, _i64, _pd]
+        L.rgo_grid_boundary.restype = _i64
+        L.rgo_grid_boundary.argtypes = [_pd, _pd, _i64, _i64, _pd, _pd]
+        for name in ("rgo_index_of_point_brute", "rgo_index_of_point_secant"):
+            f = getattr(L, name)
+            f.restype = None
+            f.argtypes = [_dbl, _dbl, _pd, _pd, _i64, _i64, _pi]
+        L.rgo_sweep_2d.restype = _i64
+        L.rgo_sweep_2d.argtypes = [
+            _pd, _pd, _i64, _i64, _pd, _pd, _i64, _i64, _pd,
+            ctypes.POINTER(_pi), ctypes.POINTER(_pi), ctypes.POINTER(_pd),
+        ]
+        L.rgo_free.restype = None
+        L.rgo_free.argtypes = [ctypes.c_void_p]
+        L.rgo_reduce_segment.restype = _dbl
+        L.rgo_reduce_segment.argtypes = [_pd, _i64]
+        L.rgo_coalesce.restype = _i64
+        L.rgo_coalesce.argtypes = [_i64, _pi, _pi, _pd, _pi, _pi, _pd]
+        L.rgo_apply.restype = None
+        L.rgo_apply.argtypes = [_i64, _pi, _pi, _pd, _i64, _pd, _i64, _pd, _i64]
+        for name in ("rgo_find_indices_brute_1d", "rgo_find_indices_searchsorted_1d"):
+            f = getattr(L, name)
+            f.restype = None
+            f.argtypes = [_pd, _i64, _pd, _i64, _i64, _i64, _pi]
+        L.rgo_set_threads.restype = None
+        L.rgo_set_threads.argtypes = [ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _d(a: np.ndarray):
+    return a.ctypes.data_as(_pd)
+
+
+def _i(a: np.ndarray):
+    return a.ctypes.data_as(_pi)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def set_threads(n: int) -> None:
+    lib().rgo_set_threads(int(n))
+
+
+def bbox_intersect(*v: float) -> bool:
+    return bool(lib().rgo_bbox_intersect(*[float(q) for q in v]))
+
+
+def seg_params(p1, p2, q1, q2) -> tuple[float, float]:
+    pq = _f64([*p1, *p2, *q1, *q2])
+    tu = np.zeros(2)
+    lib().rgo_seg_params(_d(pq), _d(tu))
+    return float(tu[0]), float(tu[1])
+
+
+def seg_hit(t: float, u: float) -> bool:
+    return bool(lib().rgo_seg_hit(float(t), float(u)))
+
+
+def point_in_polygon(x: float, y: float, vx, vy) -> bool:
+    vx = _f64(vx)
+    vy = _f64(vy)
+    return bool(lib().rgo_point_in_polygon(float(x), float(y), _d(vx), _d(vy), vx.size))
+
+
+def grid_volume(x, y) -> np.ndarray:
+    x = _f64(x)
+    y = _f64(y)
+    nx, ny = x.shape
+    out = np.empty((nx - 1, ny - 1))
+    lib().rgo_grid_volume(_d(x), _d(y), nx, ny, _d(out))
+    return out
+
+
+def grid_boundary(x, y) -> tuple[np.ndarray, np.ndarray]:
+    x = _f64(x)
+    y = _f64(y)
+    nx, ny = x.shape
+    n = 2 * (nx - 1) + 2 * (ny - 1)
+    bx = np.empty(n)
+    by = np.empty(n)
+    lib().rgo_grid_boundary(_d(x), _d(y), nx, ny, _d(bx), _d(by))
+    return bx, by
+
+
+def _index_of_point(name: str, point, grid) -> tuple[int, int]:
+    x = _f64(grid[0])
+    y = _f64(grid[1])
+    nx, ny = x.shape
+    ij = np.zeros(2, dtype=np.int64)
+    getattr(lib(), name)(float(point[0]), float(point[1]), _d(x), _d(y), nx, ny, _i(ij))
+    return int(ij[0]), int(ij[1])
+
+
+def index_of_point_brute(point, grid) -> tuple[int, int]:
+    return _index_of_point("rgo_index_of_point_brute", point, grid)
+
+
+def index_of_point_secant(point, grid) -> tuple[int, int]:
+    return _index_of_point("rgo_index_of_point_secant", point, grid)
+
+
+def sweep_2d(grid_input, grid_output, weights_input=None):
+    """Uncoalesced ``(in, out, w)`` triples in the reference's emission order."""
+    xi = _f64(grid_input[0])
+    yi = _f64(grid_input[1])
+    xo = _f64(grid_output[0])
+    yo = _f64(grid_output[1])
+    nxi, nyi = xi.shape
+    nxo, nyo = xo.shape
+    wi = None if weights_input is None else _f64(weights_input)
+    pin = _pi()
+    pout = _pi()
+    pw = _pd()
+    L = lib()
+    n = L.rgo_sweep_2d(
+        _d(xi), _d(yi), nxi, nyi, _d(xo), _d(yo), nxo, nyo,
+        None if wi is None else _d(wi),
+        ctypes.byref(pin), ctypes.byref(pout), ctypes.byref(pw),
+    )
+    try:
+        if n == 0:
+            return (np.zeros(0, np.int64), np.zeros(0, np.int64), np.zeros(0))
+        a = np.ctypeslib.as_array(pin, shape=(n,)).copy()
+        b = np.ctypeslib.as_array(pout, shape=(n,)).copy()
+        c = np.ctypeslib.as_array(pw, shape=(n,)).copy()
+    finally:
+        L.rgo_free(ctypes.cast(pin, ctypes.c_void_p))
+        L.rgo_free(ctypes.cast(pout, ctypes.c_void_p))
+        L.rgo_free(ctypes.cast(pw, ctypes.c_void_p))
+    return a, b, c
+
+
+def reduce_segment(a) -> float:
+    a = _f64(a)
+    return float(lib().rgo_reduce_segment(_d(a), a.size))
+
+
+def coalesce(indices_input, indices_output, values):
+    i = np.ascontiguousarray(indices_input, dtype=np.int64)
+    o = np.ascontiguousarray(indices_output, dtype=np.int64)
+    w = _f64(values)
+    n = i.size
+    ic = np.empty(n, np.int64)
+    oc = np.empty(n, np.int64)
+    wc = np.empty(n)
+    m = lib().rgo_coalesce(n, _i(i), _i(o), _d(w), _i(ic), _i(oc), _d(wc))
+    return ic[:m].copy(), oc[:m].copy(), wc[:m].copy()
+
+
+def weights_2d(grid_input, grid_output, weights_input=None):
+    """Sweep + coalesce for one 2-D element: the canonical weight triple."""
+    return coalesce(*sweep_2d(grid_input, grid_output, weights_input))
+
+
+def apply(indices_input, indices_output, values, values_input, n_out: int):
+    """
+    ``values_input`` is ``(D, n_in)``; returns ``(D, n_out)`` float64 built by
+    the reference's sequential scatter in storage order.
+    """
+    i = np.ascontiguousarray(indices_input, dtype=np.int64)
+    o = np.ascontiguousarray(indices_output, dtype=np.int64)
+    w = _f64(values)
+    x = _f64(values_input)
+    D, n_in = x.shape
+    y = np.zeros((D, n_out))
+    lib().rgo_apply(w.size, _i(i), _i(o), _d(w), D, _d(x), n_in, _d(y), n_out)
+    return y
+
+
+def find_indices_1d(x_input, x_output, fill_value: int, method: str):
+    """``x_input`` is ``(D, m)``, ``x_output`` ``(D, n)``; returns ``(D, n)``."""
+    a = _f64(x_input)
+    v = _f64(x_output)
+    D, m = a.shape
+    _, n = v.shape
+    res = np.empty((D, n), np.int64)
+    name = {
+        "brute": "rgo_find_indices_brute_1d",
+        "searchsorted": "rgo_find_indices_searchsorted_1d",
+    }[method]
+    getattr(lib(), name)(_d(a), m, _d(v), n, D, int(fill_value), _i(res))
+    return res

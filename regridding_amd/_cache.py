@@ -1,0 +1,66 @@
+"""
+Device residency of weights returned to the host.
+
+``weights()`` must return host arrays (the reference's flat
+``(indices_input, indices_output, values)`` triples,
+``regridding/_weights/_weights.py:31-36``), but the device already holds
+them as CSR after the build.  This registry remembers, for each returned
+triple, the device handle it came from, so ``regrid_from_weights`` on those
+very arrays (or on their ``transpose_weights`` swap) applies the resident
+CSR instead of uploading and regrouping the triples again.
+
+Entries are keyed by the identity of the three arrays and die with them.
+The registered arrays are made read-only, so a cached handle can never
+disagree with the host arrays it stands for; arrays the registry has not
+seen (unpickled, hand-made, modified copies) go through the import path.
+"""
+
+from __future__ import annotations
+
+import threading
+import weakref
+
+import numpy as np
+
+from ._lib import Weights
+
+__all__ = ["register", "lookup"]
+
+_lock = threading.Lock()
+# id(values) -> (weakref(indices_input), weakref(indices_output), Weights)
+_entries: dict[int, tuple] = {}
+
+
+def _drop(key: int) -> None:
+    with _lock:
+        _entries.pop(key, None)
+
+
+def register(indices_input: np.ndarray, indices_output: np.ndarray, values: np.ndarray,
+             handle: Weights) -> None:
+    for a in (indices_input, indices_output, values):
+        a.flags.writeable = False
+    key = id(values)
+    with _lock:
+        _entries[key] = (weakref.ref(indices_input), weakref.ref(indices_output), handle)
+    weakref.finalize(values, _drop, key)
+
+
+def lookup(a: np.ndarray, b: np.ndarray, values: np.ndarray) -> tuple[Weights, bool] | None:
+    """
+    ``(handle, transposed)`` if ``(a, b, values)`` is a registered triple
+    (``transposed=False``) or its index swap (``transposed=True``).
+    """
+    with _lock:
+        e = _entries.get(id(values))
+    if e is None:
+        return None
+    ri, ro, handle = e
+    ii, io = ri(), ro()
+    if ii is None or io is None:
+        return None
+    if a is ii and b is io:
+        return handle, False
+    if a is io and b is ii:
+        return handle, True
+    return None

@@ -1,0 +1,69 @@
+"""
+Device build of conservative 2-D weights for one orthogonal element.
+
+Replaces the compiled part of ``regridding._weights._weights_conservative``
+(``_weights_conservative.py:122-139`` calling ``weights_conservative_2d``,
+``_weights_conservative_2d.py:80-126``, then ``_weights_arrays._coalesce``):
+the coordinates (already normalised and perturbed on the host) are copied to
+the device, ``rg_build_conservative_2d`` sweeps and coalesces, and the
+handle keeps both CSR layouts resident for the apply.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from ._lib import DeviceArray, Weights, check, lib, rg_tuning, RG_TUNE_TIMINGS
+
+__all__ = ["build_2d", "build_2d_device"]
+
+
+def build_2d_device(
+    xi: DeviceArray, yi: DeviceArray, shape_input: tuple[int, int],
+    xo: DeviceArray, yo: DeviceArray, shape_output: tuple[int, int],
+    wi: DeviceArray | None = None,
+    edges_per_lane: int = 0, block: int = 0, timings: bool = False,
+    stream: int = 0,
+) -> Weights:
+    """Build from device-resident vertex arrays (row-major float64)."""
+    t = rg_tuning(int(edges_per_lane), int(block), RG_TUNE_TIMINGS if timings else 0, 0)
+    h = ctypes.c_void_p()
+    check(
+        lib().rg_build_conservative_2d(
+            xi.ptr, yi.ptr, int(shape_input[0]), int(shape_input[1]),
+            xo.ptr, yo.ptr, int(shape_output[0]), int(shape_output[1]),
+            wi.ptr if wi is not None else None, ctypes.byref(t), ctypes.byref(h), stream,
+        )
+    )
+    return Weights(h.value)
+
+
+def build_2d(
+    grid_input: tuple[np.ndarray, np.ndarray],
+    grid_output: tuple[np.ndarray, np.ndarray],
+    weights_input: np.ndarray | None = None,
+    **tuning,
+) -> Weights:
+    """Build from host vertex arrays of shape ``(nx, ny)``."""
+    xi = np.ascontiguousarray(grid_input[0], dtype=np.float64)
+    yi = np.ascontiguousarray(grid_input[1], dtype=np.float64)
+    xo = np.ascontiguousarray(grid_output[0], dtype=np.float64)
+    yo = np.ascontiguousarray(grid_output[1], dtype=np.float64)
+    if xi.ndim != 2 or xo.ndim != 2 or xi.shape != yi.shape or xo.shape != yo.shape:
+        raise ValueError("2-D grids need two coordinate arrays of one 2-D shape")
+    wi = None
+    if weights_input is not None:
+        w = np.ascontiguousarray(weights_input, dtype=np.float64)
+        if w.shape != (xi.shape[0] - 1, xi.shape[1] - 1):
+            raise ValueError(
+                f"weights_input has shape {w.shape}, expected the input cell "
+                f"shape {(xi.shape[0] - 1, xi.shape[1] - 1)}"
+            )
+        wi = DeviceArray.from_host(w)
+    return build_2d_device(
+        DeviceArray.from_host(xi), DeviceArray.from_host(yi), xi.shape,
+        DeviceArray.from_host(xo), DeviceArray.from_host(yo), xo.shape,
+        wi, **tuning,
+    )

@@ -1,0 +1,140 @@
+"""
+``weights()``: the drop-in dispatcher (``regridding/_weights/_weights.py:13-196``).
+"""
+
+from __future__ import annotations
+
+from typing import Literal, Sequence
+
+import numpy as np
+
+from . import _cache, _util
+from ._conservative import build_2d
+
+__all__ = ["weights", "conservative_handles"]
+
+
+def conservative_handles(
+    coordinates_input,
+    coordinates_output,
+    axis_input=None,
+    axis_output=None,
+    weights_input=None,
+    perturb=None,
+    seed=_util.SEED_DEFAULT,
+):
+    """
+    Normalise like ``_weights_conservative`` (``_weights_conservative.py:
+    11-146``) and build every orthogonal element on the device.
+
+    Returns ``(handles, shape_values_input, shape_values_output)`` with
+    ``handles`` an object array of :class:`regridding_amd._lib.Weights`.
+    """
+    perturb = _util.resolve_perturb(perturb, coordinates_input)
+    (
+        coordinates_input,
+        coordinates_output,
+        axis_input,
+        axis_output,
+        shape_input,
+        shape_output,
+        shape_orthogonal,
+    ) = _util.normalize_coordinates(
+        coordinates_input=coordinates_input,
+        coordinates_output=coordinates_output,
+        axis_input=axis_input,
+        axis_output=axis_output,
+        perturb=perturb,
+        seed=seed,
+    )
+
+    shape_values_input = list(shape_input)
+    for ax in axis_input:
+        shape_values_input[ax] -= 1
+    shape_values_input = tuple(shape_values_input)
+
+    shape_values_output = list(shape_output)
+    for ax in axis_output:
+        shape_values_output[ax] -= 1
+    shape_values_output = tuple(shape_values_output)
+
+    if weights_input is not None:
+        weights_input = np.broadcast_to(weights_input, shape_values_input)
+
+    if len(axis_input) == 1:
+        raise NotImplementedError(
+            "1-D conservative weights are not implemented in regridding_amd yet"
+        )
+    if len(axis_input) != 2:
+        raise NotImplementedError(
+            "Regridding operations greater than 2D are not supported"
+        )
+
+    handles = np.empty(shape_orthogonal, dtype=object)
+    x_in, y_in = coordinates_input
+    x_out, y_out = coordinates_output
+    for index in np.ndindex(*shape_orthogonal):
+        sl_in = _util.orthogonal_slice(index, axis_input)
+        sl_out = _util.orthogonal_slice(index, axis_output)
+        # the reference indexes weights_input with the bare orthogonal index
+        # (_weights_conservative.py:126)
+        wi = weights_input[index] if weights_input is not None else None
+        handles[index] = build_2d((x_in[sl_in], y_in[sl_in]), (x_out[sl_out], y_out[sl_out]), wi)
+    return handles, shape_values_input, shape_values_output
+
+
+def weights(
+    coordinates_input: tuple[np.ndarray, ...],
+    coordinates_output: tuple[np.ndarray, ...],
+    axis_input: None | int | Sequence[int] = None,
+    axis_output: None | int | Sequence[int] = None,
+    weights_input: None | np.ndarray = None,
+    method: Literal["multilinear", "conservative"] = "multilinear",
+    bounds: Literal["extrapolate", "nan", "raise"] = "extrapolate",
+    perturb: None | bool = None,
+    seed: "None | int | np.random.Generator" = _util.SEED_DEFAULT,
+) -> tuple[np.ndarray, tuple[int, ...], tuple[int, ...]]:
+    """
+    Build the sparse weights relating two logically-rectangular grids.
+
+    Same signature, return value and errors as ``regridding.weights``
+    (``regridding/_weights/_weights.py:13-196``): an object array over the
+    orthogonal axes whose elements are ``(indices_input, indices_output,
+    values)`` flat arrays sorted by ``(input, output)`` with unique pairs,
+    plus the input and output value shapes.  The conservative 2-D build runs
+    on the GPU; the returned arrays stay resident there for
+    :func:`regrid_from_weights` (they are returned read-only for that
+    reason).
+    """
+    unit_weights = getattr(weights_input, "unit", None)
+    if unit_weights is not None:
+        weights_input = getattr(weights_input, "value")
+
+    if method == "multilinear":
+        raise NotImplementedError(
+            "method='multilinear' is not implemented in regridding_amd yet"
+        )
+    elif method == "conservative":
+        handles, shape_values_input, shape_values_output = conservative_handles(
+            coordinates_input=coordinates_input,
+            coordinates_output=coordinates_output,
+            axis_input=axis_input,
+            axis_output=axis_output,
+            weights_input=weights_input,
+            perturb=perturb,
+            seed=seed,
+        )
+    else:
+        raise ValueError(f"unrecognized method '{method}'")
+
+    result = np.empty(handles.shape, dtype=object)
+    flat_h = handles.reshape(-1)
+    flat_r = result.reshape(-1)
+    for k in range(flat_h.size):
+        h = flat_h[k]
+        ii, io, w = h.export()
+        _cache.register(ii, io, w, h)
+        if unit_weights is not None:
+            w = w << unit_weights
+        flat_r[k] = (ii, io, w)
+    return result, shape_values_input, shape_values_output

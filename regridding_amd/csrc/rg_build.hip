@@ -1,0 +1,1024 @@
+// rg_build.hip -- the conservative 2-D weight build on the GPU.
+//
+// Pipeline for one grid pair (all on one stream):
+//   prepare   input cell areas; per static grid: bbox, boundary polygon,
+//             boundary edge table in the outside-scan order, edge boxes
+//   start     chunk 0 of every row: the reference's row start (wavefront per
+//             row); chunk c > 0: a guessed start from the cell search
+//   count     every chunk walks, counts its triples, records its end state
+//             and adds its triples to the per-input-cell histogram
+//   fixup     per row: chunks whose start differs from the predecessor's end
+//             are walked again from the true state (histogram corrected)
+//   emit      scans give each chunk its first emission ordinal and each
+//             input cell its bucket; every chunk walks again and writes
+//             (out << 32 | ordinal, w) straight into the buckets
+//   reduce    per input cell: sort the bucket, reduce duplicate pairs in
+//             np.add.reduceat order -> CSR by input (the reference order)
+//   transpose CSR by output (columns ascending) for the forward apply
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <string>
+
+#include "rg_internal.hpp"
+#include "rg_sweep.hpp"
+#include "rg_wave.hpp"
+
+namespace rg {
+
+// ---- atomics usable from the shared walk code ----------------------------
+
+struct CountSink {
+    int32_t n;
+    int32_t* hist;  // per input cell, or null
+    __device__ void emit(int64_t cin, int64_t, double)
+    {
+        n++;
+        if (hist) atomicAdd(&hist[cin], 1);
+    }
+};
+
+struct UncountSink {
+    int32_t* hist;
+    __device__ void emit(int64_t cin, int64_t, double)
+    {
+        atomicAdd(&hist[cin], -1);
+    }
+};
+
+// Writes each triple into its input cell's bucket.  The emit walk repeats
+// the (fixed-up) count walk, so a bucket never receives more than its
+// histogram count; the bound check only turns a broken invariant into a
+// reported error instead of an out-of-bounds store.
+struct BucketSink {
+    uint32_t ordinal;
+    const int64_t* __restrict__ bucket_off;
+    int32_t* __restrict__ cursor;
+    uint64_t* __restrict__ key;
+    double* __restrict__ val;
+    int32_t* __restrict__ overflow;
+    __device__ void emit(int64_t cin, int64_t cout, double w)
+    {
+        int64_t pos = bucket_off[cin] + atomicAdd(&cursor[cin], 1);
+        if (pos < bucket_off[cin + 1]) {
+            key[pos] = ((uint64_t)cout << 32) | (uint64_t)ordinal;
+            val[pos] = w;
+        } else {
+            atomicAdd(overflow, 1);
+        }
+        ordinal++;
+    }
+};
+
+// ---- prepare -------------------------------------------------------------
+
+__global__ void k_area(const double* __restrict__ x, const double* __restrict__ y,
+                       int64_t nx, int64_t ny, double* __restrict__ out)
+{
+    const int64_t ncy = ny - 1;
+    const int64_t n = (nx - 1) * ncy;
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        out[c] = cell_area(x, y, ny, c / ncy, c % ncy);
+    }
+}
+
+int launch_area(const double* x, const double* y, int64_t nx, int64_t ny,
+                double* out, hipStream_t s)
+{
+    const int64_t n = (nx - 1) * (ny - 1);
+    if (n <= 0) return RG_OK;
+    k_area<<<std::min<unsigned>(grid_for(n, 256), 65536), 256, 0, s>>>(x, y, nx,
+                                                                       ny, out);
+    RG_HIP(hipGetLastError());
+    return RG_OK;
+}
+
+// numpy min/max semantics: a NaN anywhere makes the result NaN.
+__device__ inline double nan_min(double a, double b)
+{
+    return (a != a || b != b) ? __builtin_nan("") : (a < b ? a : b);
+}
+__device__ inline double nan_max(double a, double b)
+{
+    return (a != a || b != b) ? __builtin_nan("") : (a > b ? a : b);
+}
+
+constexpr int BBOX_BLOCK = 256;
+
+__global__ void k_bbox_partial(const double* __restrict__ x,
+                               const double* __restrict__ y, int64_t n,
+                               double* __restrict__ part)
+{
+    double v[4] = {x[0], y[0], x[0], y[0]};
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        v[0] = nan_min(v[0], x[i]);
+        v[1] = nan_min(v[1], y[i]);
+        v[2] = nan_max(v[2], x[i]);
+        v[3] = nan_max(v[3], y[i]);
+    }
+    __shared__ double sh[4][BBOX_BLOCK];
+    for (int q = 0; q < 4; q++) sh[q][threadIdx.x] = v[q];
+    __syncthreads();
+    for (int o = BBOX_BLOCK / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            sh[0][threadIdx.x] = nan_min(sh[0][threadIdx.x], sh[0][threadIdx.x + o]);
+            sh[1][threadIdx.x] = nan_min(sh[1][threadIdx.x], sh[1][threadIdx.x + o]);
+            sh[2][threadIdx.x] = nan_max(sh[2][threadIdx.x], sh[2][threadIdx.x + o]);
+            sh[3][threadIdx.x] = nan_max(sh[3][threadIdx.x], sh[3][threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 4; q++) part[4 * blockIdx.x + q] = sh[q][0];
+}
+
+__global__ void k_bbox_final(const double* __restrict__ part, int nparts,
+                             double* __restrict__ bbox)
+{
+    if (threadIdx.x != 0) return;
+    double v[4] = {part[0], part[1], part[2], part[3]};
+    for (int b = 1; b < nparts; b++) {
+        v[0] = nan_min(v[0], part[4 * b + 0]);
+        v[1] = nan_min(v[1], part[4 * b + 1]);
+        v[2] = nan_max(v[2], part[4 * b + 2]);
+        v[3] = nan_max(v[3], part[4 * b + 3]);
+    }
+    for (int q = 0; q < 4; q++) bbox[q] = v[q];
+}
+
+// Boundary polygon and boundary edge table (same length 2(nx-1)+2(ny-1)).
+__global__ void k_boundary(const double* __restrict__ x,
+                           const double* __restrict__ y, int64_t nx, int64_t ny,
+                           double* __restrict__ bpx, double* __restrict__ bpy,
+                           double* __restrict__ be, int32_t* __restrict__ bcell)
+{
+    const int64_t n = 2 * (nx - 1) + 2 * (ny - 1);
+    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    int64_t v = boundary_vertex(nx, ny, e);
+    bpx[e] = x[v];
+    bpy[e] = y[v];
+    boundary_edge(x, y, nx, ny, e, be + 4 * e, bcell + e);
+}
+
+__global__ void k_edge_boxes(const double* __restrict__ be, int64_t nbe,
+                             Box* __restrict__ bblk, int64_t nblk)
+{
+    const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (b >= nblk) return;
+    Box r;
+    r.x0 = r.y0 = __builtin_huge_val();
+    r.x1 = r.y1 = -__builtin_huge_val();
+    int64_t e1 = (b + 1) * BND_BLOCK < nbe ? (b + 1) * BND_BLOCK : nbe;
+    for (int64_t e = b * BND_BLOCK; e < e1; e++) {
+        const double* q = be + 4 * e;
+        // fmin/fmax ignore NaN; an edge with a NaN end never hits anyway
+        r.x0 = fmin(r.x0, fmin(q[0], q[2]));
+        r.x1 = fmax(r.x1, fmax(q[0], q[2]));
+        r.y0 = fmin(r.y0, fmin(q[1], q[3]));
+        r.y1 = fmax(r.y1, fmax(q[1], q[3]));
+    }
+    bblk[b] = r;
+}
+
+__global__ void k_super_boxes(const Box* __restrict__ bblk, int64_t nblk,
+                              Box* __restrict__ bsup, int64_t nsup)
+{
+    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= nsup) return;
+    Box r;
+    r.x0 = r.y0 = __builtin_huge_val();
+    r.x1 = r.y1 = -__builtin_huge_val();
+    int64_t b1 = (s + 1) * BND_SUPER < nblk ? (s + 1) * BND_SUPER : nblk;
+    for (int64_t b = s * BND_SUPER; b < b1; b++) {
+        r.x0 = fmin(r.x0, bblk[b].x0);
+        r.y0 = fmin(r.y0, bblk[b].y0);
+        r.x1 = fmax(r.x1, bblk[b].x1);
+        r.y1 = fmax(r.y1, bblk[b].y1);
+    }
+    bsup[s] = r;
+}
+
+// ---- start states --------------------------------------------------------
+
+// Row start of the reference (:293-322), one wavefront per row: bounding
+// box, winding number over the static boundary split across the lanes,
+// secant search on lane 0 with the exhaustive fallback spread over the wave.
+__global__ void k_row_start(Build B, int32_t* __restrict__ start,
+                            int32_t* __restrict__ anomalies)
+{
+    const int64_t gw = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (gw >= B.nrows_total) return;  // uniform per wavefront
+    const int p = pass_of_row(B, gw);
+    const Pass& P = B.pass[p];
+    const StaticGrid& s = P.st;
+    const int64_t row = gw - B.row_base[p];
+    const int64_t g = P.chunk_base + row * P.cpr;
+    const double px = sweep_x(P, row, 0), py = sweep_y(P, row, 0);
+    int32_t state = STATE_OUTSIDE;
+    if (s.bbox[0] <= px && px <= s.bbox[2] && s.bbox[1] <= py &&
+        py <= s.bbox[3]) {
+        double w = 0;
+        for (int64_t k = lane; k < s.nbp; k += 64) {
+            int64_t km = k == 0 ? s.nbp - 1 : k - 1;
+            w += winding_term(px, py, s.bpx[km], s.bpy[km], s.bpx[k], s.bpy[k]);
+        }
+        w = wave_sum(w);
+        if (w != 0) {
+            int64_t c = 0;
+            if (lane == 0) c = index_secant(s.x, s.y, s.nx, s.ny, px, py);
+            c = __shfl(c, 0, 64);
+            if (c == RG_SEARCH_BRUTE) c = wave_brute(s.x, s.y, s.nx, s.ny, px, py, lane);
+            if (c >= 0) {
+                state = (int32_t)c;
+            } else if (lane == 0) {
+                // inside the boundary polygon but in no cell: the reference
+                // then indexes with sys.maxsize; treated as outside here
+                atomicAdd(anomalies, 1);
+            }
+        }
+    }
+    if (lane == 0) start[g] = state;
+}
+
+__global__ void k_chunk_start(Build B, int32_t* __restrict__ start)
+{
+    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (g >= B.nchunks) return;
+    const int p = pass_of_chunk(B, g);
+    const Pass& P = B.pass[p];
+    const int64_t local = g - P.chunk_base;
+    const int64_t row = local / P.cpr, c = local % P.cpr;
+    if (c == 0) return;
+    start[g] = chunk_start_guess(P, row, c * B.K);
+}
+
+// ---- walks ---------------------------------------------------------------
+
+__device__ inline void chunk_range(const Build& B, const Pass& P, int64_t c,
+                                   int64_t* k0, int64_t* k1)
+{
+    *k0 = c * B.K;
+    int64_t e = *k0 + B.K;
+    *k1 = e < P.nk - 1 ? e : P.nk - 1;
+    if (*k1 < *k0) *k1 = *k0;
+}
+
+__global__ void k_walk_count(Build B, const int32_t* __restrict__ start,
+                             int32_t* __restrict__ end,
+                             int32_t* __restrict__ count,
+                             int32_t* __restrict__ hist)
+{
+    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (g >= B.nchunks) return;
+    const int p = pass_of_chunk(B, g);
+    const Pass& P = B.pass[p];
+    const int64_t local = g - P.chunk_base;
+    const int64_t row = local / P.cpr, c = local % P.cpr;
+    int64_t k0, k1;
+    chunk_range(B, P, c, &k0, &k1);
+    CountSink sink{0, hist};
+    end[g] = walk_chunk(B, P, row, k0, k1, start[g], sink);
+    count[g] = sink.n;
+}
+
+// One thread per row: make every chunk start where its predecessor ended.
+__global__ void k_fixup(Build B, int32_t* __restrict__ start,
+                        int32_t* __restrict__ end, int32_t* __restrict__ count,
+                        int32_t* __restrict__ hist, int32_t* __restrict__ nfix,
+                        int32_t* __restrict__ failed)
+{
+    const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (r >= B.nrows_total) return;
+    const int p = pass_of_row(B, r);
+    const Pass& P = B.pass[p];
+    const int64_t row = r - B.row_base[p];
+    const int64_t g0 = P.chunk_base + row * P.cpr;
+    int32_t fixes = 0;
+    for (int64_t c = 1; c < P.cpr; c++) {
+        const int64_t g = g0 + c;
+        const int32_t truth = end[g - 1];
+        if (truth == STATE_FAILED) break;
+        if (start[g] == truth) continue;
+        int64_t k0, k1;
+        chunk_range(B, P, c, &k0, &k1);
+        UncountSink undo{hist};
+        (void)walk_chunk(B, P, row, k0, k1, start[g], undo);
+        CountSink redo{0, hist};
+        end[g] = walk_chunk(B, P, row, k0, k1, truth, redo);
+        count[g] = redo.n;
+        start[g] = truth;
+        fixes++;
+    }
+    for (int64_t c = 0; c < P.cpr; c++)
+        if (end[g0 + c] == STATE_FAILED) {
+            atomicAdd(failed, 1);
+            break;
+        }
+    if (fixes) atomicAdd(nfix, fixes);
+}
+
+__global__ void k_walk_emit(Build B, const int32_t* __restrict__ start,
+                            const int64_t* __restrict__ chunk_off,
+                            const int64_t* __restrict__ bucket_off,
+                            int32_t* __restrict__ cursor,
+                            uint64_t* __restrict__ key, double* __restrict__ val,
+                            int32_t* __restrict__ overflow)
+{
+    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (g >= B.nchunks) return;
+    const int p = pass_of_chunk(B, g);
+    const Pass& P = B.pass[p];
+    const int64_t local = g - P.chunk_base;
+    const int64_t row = local / P.cpr, c = local % P.cpr;
+    int64_t k0, k1;
+    chunk_range(B, P, c, &k0, &k1);
+    BucketSink sink{(uint32_t)chunk_off[g], bucket_off, cursor, key, val,
+                    overflow};
+    (void)walk_chunk(B, P, row, k0, k1, start[g], sink);
+}
+
+// ---- reduce and CSR ------------------------------------------------------
+
+__global__ void k_bucket_reduce(const int64_t* __restrict__ bucket_off,
+                                uint64_t* __restrict__ key,
+                                double* __restrict__ val, int64_t n_in,
+                                int32_t* __restrict__ nnz_row,
+                                int32_t* __restrict__ long_list,
+                                int32_t* __restrict__ n_long)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n_in) return;
+    const int64_t s = bucket_off[i], e = bucket_off[i + 1];
+    bool lr = false;
+    nnz_row[i] = (int32_t)bucket_reduce(key + s, val + s, e - s, &lr);
+    if (lr) long_list[atomicAdd(n_long, 1)] = (int32_t)i;
+}
+
+// Buckets with a run of more than 129 terms (see bucket_reduce); a
+// grid-stride loop over the (normally empty) list.
+__global__ void k_bucket_long(const int64_t* __restrict__ bucket_off,
+                              uint64_t* __restrict__ key,
+                              double* __restrict__ val,
+                              const int32_t* __restrict__ long_list,
+                              const int32_t* __restrict__ n_long)
+{
+    const int32_t nl = *n_long;
+    for (int32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nl;
+         q += gridDim.x * blockDim.x) {
+        const int64_t i = long_list[q];
+        const int64_t s = bucket_off[i], e = bucket_off[i + 1];
+        (void)bucket_reduce_long(key + s, val + s, e - s);
+    }
+}
+
+__global__ void k_csr_in(const int64_t* __restrict__ bucket_off,
+                         const int64_t* __restrict__ row_off,
+                         const uint64_t* __restrict__ key,
+                         const double* __restrict__ val, int64_t n_in,
+                         int32_t* __restrict__ rowptr,
+                         int32_t* __restrict__ col, double* __restrict__ out_val,
+                         int32_t* __restrict__ hist_out)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i > n_in) return;
+    rowptr[i] = (int32_t)row_off[i];
+    if (i == n_in) return;
+    const int64_t s = bucket_off[i];
+    const int64_t d = row_off[i], m = row_off[i + 1] - d;
+    for (int64_t q = 0; q < m; q++) {
+        int32_t o = (int32_t)key[s + q];
+        col[d + q] = o;
+        out_val[d + q] = val[s + q];
+        atomicAdd(&hist_out[o], 1);
+    }
+}
+
+// Transpose scatter: entry q of row i of a CSR goes to row col[q] of the
+// transposed CSR at an atomically claimed slot; k_rows_sort then orders each
+// transposed row by column, which makes the result deterministic.
+__global__ void k_transpose_scatter(const int32_t* __restrict__ rowptr,
+                                    const int32_t* __restrict__ col,
+                                    const double* __restrict__ val,
+                                    int64_t nrows,
+                                    const int64_t* __restrict__ t_off,
+                                    int32_t* __restrict__ cursor,
+                                    int32_t* __restrict__ t_col,
+                                    double* __restrict__ t_val)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= nrows) return;
+    for (int32_t q = rowptr[i]; q < rowptr[i + 1]; q++) {
+        int32_t o = col[q];
+        int64_t pos = t_off[o] + atomicAdd(&cursor[o], 1);
+        if (pos < t_off[o + 1]) {
+            t_col[pos] = (int32_t)i;
+            t_val[pos] = val[q];
+        }
+    }
+}
+
+__global__ void k_rows_sort(const int64_t* __restrict__ off, int64_t nrows,
+                            int32_t* __restrict__ col, double* __restrict__ val,
+                            int32_t* __restrict__ rowptr)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i > nrows) return;
+    rowptr[i] = (int32_t)off[i];
+    if (i == nrows) return;
+    const int64_t s = off[i];
+    sort_pairs(col + s, val + s, off[i + 1] - s);
+}
+
+__global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// ---- host orchestration --------------------------------------------------
+
+namespace {
+
+struct StaticTables {
+    DevBuf<double> bbox, part, bpx, bpy, be;
+    DevBuf<int32_t> bcell;
+    DevBuf<Box> bblk, bsup;
+};
+
+int fill_zero(int32_t* p, int64_t n, hipStream_t s)
+{
+    if (n == 0) return RG_OK;
+    k_fill_i32<<<grid_for(n, 256) > 4096 ? 4096 : grid_for(n, 256), 256, 0,
+                 s>>>(p, 0, n);
+    RG_HIP(hipGetLastError());
+    return RG_OK;
+}
+
+int prepare_static(const double* x, const double* y, int64_t nx, int64_t ny,
+                   StaticTables& T, StaticGrid& G, hipStream_t s)
+{
+    const int64_t nv = nx * ny;
+    const int64_t nbe = 2 * (nx - 1) + 2 * (ny - 1);
+    const int64_t nblk = (nbe + BND_BLOCK - 1) / BND_BLOCK;
+    const int64_t nsup = (nblk + BND_SUPER - 1) / BND_SUPER;
+    int nparts = (int)std::min<int64_t>(grid_for(nv, BBOX_BLOCK), 1024);
+    RG_TRY(T.bbox.alloc(4, s));
+    RG_TRY(T.part.alloc(4 * (size_t)nparts, s));
+    RG_TRY(T.bpx.alloc(nbe, s));
+    RG_TRY(T.bpy.alloc(nbe, s));
+    RG_TRY(T.be.alloc(4 * nbe, s));
+    RG_TRY(T.bcell.alloc(nbe, s));
+    RG_TRY(T.bblk.alloc(nblk, s));
+    RG_TRY(T.bsup.alloc(nsup, s));
+    k_bbox_partial<<<nparts, BBOX_BLOCK, 0, s>>>(x, y, nv, T.part.p);
+    RG_HIP(hipGetLastError());
+    k_bbox_final<<<1, 64, 0, s>>>(T.part.p, nparts, T.bbox.p);
+    RG_HIP(hipGetLastError());
+    k_boundary<<<grid_for(nbe, 256), 256, 0, s>>>(x, y, nx, ny, T.bpx.p,
+                                                   T.bpy.p, T.be.p, T.bcell.p);
+    RG_HIP(hipGetLastError());
+    k_edge_boxes<<<grid_for(nblk, 256), 256, 0, s>>>(T.be.p, nbe, T.bblk.p, nblk);
+    RG_HIP(hipGetLastError());
+    k_super_boxes<<<grid_for(nsup, 256), 256, 0, s>>>(T.bblk.p, nblk, T.bsup.p,
+                                                       nsup);
+    RG_HIP(hipGetLastError());
+    G.x = x;
+    G.y = y;
+    G.nx = nx;
+    G.ny = ny;
+    G.bbox = T.bbox.p;
+    G.bpx = T.bpx.p;
+    G.bpy = T.bpy.p;
+    G.nbp = nbe;
+    G.be = T.be.p;
+    G.bcell = T.bcell.p;
+    G.nbe = nbe;
+    G.bblk = T.bblk.p;
+    G.nblk = nblk;
+    G.bsup = T.bsup.p;
+    G.nsup = nsup;
+    return RG_OK;
+}
+
+struct PhaseTimer {
+    bool on;
+    hipStream_t s;
+    hipEvent_t ev[10];
+    int n = 0;
+    explicit PhaseTimer(bool enabled, hipStream_t st) : on(enabled), s(st)
+    {
+        if (on)
+            for (auto& e : ev) (void)hipEventCreate(&e);
+    }
+    ~PhaseTimer()
+    {
+        if (on)
+            for (auto& e : ev) (void)hipEventDestroy(e);
+    }
+    void mark()
+    {
+        if (on && n < 10) (void)hipEventRecord(ev[n++], s);
+    }
+    double ms(int a, int b)
+    {
+        if (!on || b >= n) return 0;
+        (void)hipEventSynchronize(ev[b]);
+        float t = 0;
+        (void)hipEventElapsedTime(&t, ev[a], ev[b]);
+        return t;
+    }
+};
+
+}  // namespace
+
+int build_conservative_2d(const double* xi, const double* yi, int64_t nxi,
+                          int64_t nyi, const double* xo, const double* yo,
+                          int64_t nxo, int64_t nyo, const double* wi,
+                          const rg_tuning* tuning, rg_weights** out,
+                          hipStream_t s)
+{
+    *out = nullptr;
+    if (injected_failure("build"))
+        return fail(RG_ERR_INJECTED, "injected build failure");
+    if (!xi || !yi || !xo || !yo)
+        return fail(RG_ERR_ARGUMENT, "coordinate pointer is NULL");
+    if (nxi < 2 || nyi < 2 || nxo < 2 || nyo < 2)
+        return fail(RG_ERR_ARGUMENT,
+                    "each grid needs at least 2 vertices along both axes");
+    const int64_t n_in = (nxi - 1) * (nyi - 1), n_out = (nxo - 1) * (nyo - 1);
+    if (n_in >= INT32_MAX || n_out >= INT32_MAX)
+        return fail(RG_ERR_ARGUMENT, "more than 2^31 - 1 cells in one grid");
+    int64_t K = 16;
+    int block = 128;
+    bool timings = false;
+    if (tuning) {
+        if (tuning->edges_per_lane > 0) K = tuning->edges_per_lane;
+        if (tuning->block > 0) block = tuning->block;
+        timings = (tuning->flags & RG_TUNE_TIMINGS) != 0;
+    }
+    if (block % 64 != 0 || block > 1024)
+        return fail(RG_ERR_ARGUMENT, "block must be a multiple of 64, <= 1024");
+
+    auto t_host0 = std::chrono::steady_clock::now();
+    PhaseTimer tm(timings, s);
+    tm.mark();  // 0
+
+    // ---- prepare
+    DevBuf<double> vol;
+    RG_TRY(vol.alloc(n_in, s));
+    RG_TRY(launch_area(xi, yi, nxi, nyi, vol.p, s));
+    StaticTables tab_in, tab_out;
+    StaticGrid st_in, st_out;
+    RG_TRY(prepare_static(xi, yi, nxi, nyi, tab_in, st_in, s));
+    RG_TRY(prepare_static(xo, yo, nxo, nyo, tab_out, st_out, s));
+
+    Build B;
+    B.vol_in = vol.p;
+    B.wi = wi;
+    B.K = K;
+    int64_t chunk = 0, rows = 0;
+    for (int p = 0; p < 4; p++) {
+        Pass& P = B.pass[p];
+        const int sweep_input = p >= 2;
+        const int axis = p & 1;
+        const double* xs = sweep_input ? xi : xo;
+        const double* ys = sweep_input ? yi : yo;
+        const int64_t nxs = sweep_input ? nxi : nxo;
+        const int64_t nys = sweep_input ? nyi : nyo;
+        P.xs = xs;
+        P.ys = ys;
+        if (axis == 0) {  // aligned view = transpose
+            P.sr = 1; P.sk = nys; P.nrows = nys; P.nk = nxs;
+        } else {
+            P.sr = nys; P.sk = 1; P.nrows = nxs; P.nk = nys;
+        }
+        P.ncy_sweep = nys - 1;
+        P.axis = axis;
+        P.sweep_input = sweep_input;
+        P.st = sweep_input ? st_out : st_in;
+        P.cpr = std::max<int64_t>(1, (P.nk - 1 + K - 1) / K);
+        P.chunk_base = chunk;
+        B.row_base[p] = rows;
+        chunk += P.nrows * P.cpr;
+        rows += P.nrows;
+    }
+    B.nchunks = chunk;
+    B.nrows_total = rows;
+    if (B.nchunks >= INT32_MAX)
+        return fail(RG_ERR_ARGUMENT, "too many sweep chunks");
+    tm.mark();  // 1
+
+    // ---- start states
+    DevBuf<int32_t> start, end, count, hist, counters;
+    RG_TRY(start.alloc(B.nchunks, s));
+    RG_TRY(end.alloc(B.nchunks, s));
+    RG_TRY(count.alloc(B.nchunks, s));
+    RG_TRY(hist.alloc(n_in, s));
+    RG_TRY(counters.alloc(4, s));  // anomalies, fixups, failed, long runs
+    RG_TRY(fill_zero(hist.p, n_in, s));
+    RG_TRY(fill_zero(counters.p, 4, s));
+    k_row_start<<<grid_for(B.nrows_total * 64, 256), 256, 0, s>>>(
+        B, start.p, counters.p + 0);
+    RG_HIP(hipGetLastError());
+    k_chunk_start<<<grid_for(B.nchunks, block), block, 0, s>>>(B, start.p);
+    RG_HIP(hipGetLastError());
+    tm.mark();  // 2
+
+    // ---- count
+    k_walk_count<<<grid_for(B.nchunks, block), block, 0, s>>>(
+        B, start.p, end.p, count.p, hist.p);
+    RG_HIP(hipGetLastError());
+    tm.mark();  // 3
+
+    // ---- fix-up
+    k_fixup<<<grid_for(B.nrows_total, 64), 64, 0, s>>>(
+        B, start.p, end.p, count.p, hist.p, counters.p + 1, counters.p + 2);
+    RG_HIP(hipGetLastError());
+    DevBuf<int64_t> chunk_off, bucket_off;
+    RG_TRY(chunk_off.alloc(B.nchunks + 1, s));
+    RG_TRY(bucket_off.alloc(n_in + 1, s));
+    RG_TRY(scan_i32_to_i64(count.p, chunk_off.p, B.nchunks, s));
+    RG_TRY(scan_i32_to_i64(hist.p, bucket_off.p, n_in, s));
+    int64_t tot[2];
+    int32_t cnt[4];
+    RG_TRY(read_scalars(chunk_off.p + B.nchunks, &tot[0], sizeof(int64_t), s));
+    RG_TRY(read_scalars(bucket_off.p + n_in, &tot[1], sizeof(int64_t), s));
+    RG_TRY(read_scalars(counters.p, cnt, sizeof(cnt), s));
+    tm.mark();  // 4
+    if (cnt[2] != 0)
+        return fail(RG_ERR_GEOMETRY,
+                    "the sweep did not terminate in " + std::to_string(cnt[2]) +
+                        " rows (exactly degenerate crossings)");
+    const int64_t T = tot[0];
+    if (T != tot[1])
+        return fail(RG_ERR_HIP, "internal: triple count " + std::to_string(T) +
+                                    " != histogram total " +
+                                    std::to_string(tot[1]));
+    if (T >= ((int64_t)1 << 32))
+        return fail(RG_ERR_ARGUMENT, "more than 2^32 triples in one element");
+
+    // ---- emit
+    DevBuf<uint64_t> key;
+    DevBuf<double> val;
+    RG_TRY(key.alloc(T, s));
+    RG_TRY(val.alloc(T, s));
+    RG_TRY(fill_zero(hist.p, n_in, s));  // reused as the bucket cursor
+    k_walk_emit<<<grid_for(B.nchunks, block), block, 0, s>>>(
+        B, start.p, chunk_off.p, bucket_off.p, hist.p, key.p, val.p,
+        counters.p + 2);
+    RG_HIP(hipGetLastError());
+    tm.mark();  // 5
+
+    // ---- reduce -> CSR by input
+    DevBuf<int32_t> nnz_row, long_list;
+    DevBuf<int64_t> row_off;
+    RG_TRY(nnz_row.alloc(n_in, s));
+    RG_TRY(long_list.alloc(n_in, s));
+    RG_TRY(row_off.alloc(n_in + 1, s));
+    k_bucket_reduce<<<grid_for(n_in, 128), 128, 0, s>>>(
+        bucket_off.p, key.p, val.p, n_in, nnz_row.p, long_list.p,
+        counters.p + 3);
+    k_bucket_long<<<4, 64, 0, s>>>(bucket_off.p, key.p, val.p, long_list.p,
+                                   counters.p + 3);
+    RG_HIP(hipGetLastError());
+    RG_TRY(scan_i32_to_i64(nnz_row.p, row_off.p, n_in, s));
+    int64_t nnz = 0;
+    RG_TRY(read_scalars(row_off.p + n_in, &nnz, sizeof(int64_t), s));
+    int32_t cnt2[4];
+    RG_TRY(read_scalars(counters.p, cnt2, sizeof(cnt2), s));
+    if (cnt2[2] != 0)
+        return fail(RG_ERR_HIP, "internal: emission overflowed its buckets (" +
+                                    std::to_string(cnt2[2]) + " triples)");
+    if (nnz >= INT32_MAX)
+        return fail(RG_ERR_ARGUMENT, "more than 2^31 - 1 weights in one element");
+
+    rg_weights* W = new rg_weights();
+    W->n_in = n_in;
+    W->n_out = n_out;
+    W->nnz = nnz;
+    (void)hipGetDevice(&W->device);
+    DevBuf<int32_t> rp_in, c_in, rp_out, c_out, hist_out;
+    DevBuf<double> v_in, v_out;
+    DevBuf<int64_t> off_out;
+    int st = RG_OK;
+    do {
+        if ((st = rp_in.alloc(n_in + 1, s))) break;
+        if ((st = c_in.alloc(nnz, s))) break;
+        if ((st = v_in.alloc(nnz, s))) break;
+        if ((st = rp_out.alloc(n_out + 1, s))) break;
+        if ((st = c_out.alloc(nnz, s))) break;
+        if ((st = v_out.alloc(nnz, s))) break;
+        if ((st = hist_out.alloc(n_out, s))) break;
+        if ((st = off_out.alloc(n_out + 1, s))) break;
+        if ((st = fill_zero(hist_out.p, n_out, s))) break;
+        k_csr_in<<<grid_for(n_in + 1, 128), 128, 0, s>>>(
+            bucket_off.p, row_off.p, key.p, val.p, n_in, rp_in.p, c_in.p,
+            v_in.p, hist_out.p);
+        if (hipGetLastError() != hipSuccess) {
+            st = fail(RG_ERR_HIP, "k_csr_in launch failed");
+            break;
+        }
+        tm.mark();  // 6
+        // ---- transpose -> CSR by output
+        if ((st = scan_i32_to_i64(hist_out.p, off_out.p, n_out, s))) break;
+        if ((st = fill_zero(hist_out.p, n_out, s))) break;  // cursor
+        k_transpose_scatter<<<grid_for(n_in, 128), 128, 0, s>>>(
+            rp_in.p, c_in.p, v_in.p, n_in, off_out.p, hist_out.p, c_out.p,
+            v_out.p);
+        k_rows_sort<<<grid_for(n_out + 1, 128), 128, 0, s>>>(
+            off_out.p, n_out, c_out.p, v_out.p, rp_out.p);
+        if (hipGetLastError() != hipSuccess) {
+            st = fail(RG_ERR_HIP, "transpose launch failed");
+            break;
+        }
+        tm.mark();  // 7
+    } while (0);
+    if (st != RG_OK) {
+        delete W;
+        return st;
+    }
+    W->rowptr_in = rp_in.release();
+    W->col_in = c_in.release();
+    W->val_in = v_in.release();
+    W->rowptr_out = rp_out.release();
+    W->col_out = c_out.release();
+    W->val_out = v_out.release();
+    rg_build_stats& S = W->stats;
+    S.n_triples = T;
+    S.n_chunks = B.nchunks;
+    S.n_fixups = cnt[1];
+    S.n_search_anomalies = cnt[0];
+    // every inside sub-segment emits two triples except on the first and
+    // last row of a sweep, which emit one
+    S.n_subsegments = T / 2;
+    if (timings) {
+        S.ms_prepare = tm.ms(0, 1);
+        S.ms_start = tm.ms(1, 2);
+        S.ms_count = tm.ms(2, 3);
+        S.ms_fixup = tm.ms(3, 4);
+        S.ms_emit = tm.ms(4, 5);
+        S.ms_reduce = tm.ms(5, 6);
+        S.ms_transpose = tm.ms(6, 7);
+        S.ms_total = tm.ms(0, 7);
+    }
+    (void)t_host0;
+    *out = W;
+    return RG_OK;
+}
+
+void weights_release(rg_weights* W)
+{
+    if (!W) return;
+    dev_free(W->rowptr_in);
+    dev_free(W->col_in);
+    dev_free(W->val_in);
+    dev_free(W->rowptr_out);
+    dev_free(W->col_out);
+    dev_free(W->val_out);
+    dev_free(W->val_in_f32);
+    dev_free(W->val_out_f32);
+    delete W;
+}
+
+// ---- export / import -----------------------------------------------------
+
+__global__ void k_export(const int32_t* __restrict__ rowptr,
+                         const int32_t* __restrict__ col,
+                         const double* __restrict__ v, int64_t nrows,
+                         int64_t* __restrict__ oi, int64_t* __restrict__ oo,
+                         double* __restrict__ ov)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= nrows) return;
+    for (int32_t q = rowptr[i]; q < rowptr[i + 1]; q++) {
+        oi[q] = i;
+        oo[q] = col[q];
+        ov[q] = v[q];
+    }
+}
+
+// Import: group the storage-ordered triples by one index, keeping storage
+// order inside each group (positions are the sort keys).
+__global__ void k_hist_i64(const int64_t* __restrict__ idx, int64_t n,
+                           int32_t* __restrict__ hist, int32_t* __restrict__ bad,
+                           int64_t limit)
+{
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        int64_t k = idx[q];
+        if (k < 0 || k >= limit) {
+            atomicAdd(bad, 1);
+            continue;
+        }
+        atomicAdd(&hist[k], 1);
+    }
+}
+
+__global__ void k_check_range(const int64_t* __restrict__ idx, int64_t n,
+                              int32_t* __restrict__ bad, int64_t limit)
+{
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        int64_t k = idx[q];
+        if (k < 0 || k >= limit) atomicAdd(bad, 1);
+    }
+}
+
+__global__ void k_group_scatter(const int64_t* __restrict__ idx, int64_t n,
+                                const int64_t* __restrict__ off,
+                                int32_t* __restrict__ cursor,
+                                int32_t* __restrict__ pos_out)
+{
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        int64_t k = idx[q];
+        int64_t p = off[k] + atomicAdd(&cursor[k], 1);
+        if (p < off[k + 1]) pos_out[p] = (int32_t)q;
+    }
+}
+
+__global__ void k_group_finish(const int64_t* __restrict__ off, int64_t nrows,
+                               int32_t* __restrict__ pos,
+                               const int64_t* __restrict__ other,
+                               const double* __restrict__ w,
+                               int32_t* __restrict__ rowptr,
+                               int32_t* __restrict__ col,
+                               double* __restrict__ val)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i > nrows) return;
+    rowptr[i] = (int32_t)off[i];
+    if (i == nrows) return;
+    const int64_t s = off[i], n = off[i + 1] - s;
+    // order the group by storage position (insertion for short groups)
+    int32_t* p = pos + s;
+    if (n <= 48) {
+        for (int64_t a = 1; a < n; a++) {
+            int32_t k = p[a];
+            int64_t b = a - 1;
+            while (b >= 0 && p[b] > k) {
+                p[b + 1] = p[b];
+                b--;
+            }
+            p[b + 1] = k;
+        }
+    } else {
+        double* dummy = val + s;  // reuse as the carried payload
+        for (int64_t a = 0; a < n; a++) dummy[a] = 0;
+        sort_pairs(p, dummy, n);
+    }
+    for (int64_t a = 0; a < n; a++) {
+        int32_t q = p[a];
+        col[s + a] = (int32_t)other[q];
+        val[s + a] = w[q];
+    }
+}
+
+namespace {
+
+int group_csr(const int64_t* idx, const int64_t* other, const double* w,
+              int64_t nnz, int64_t nrows, int64_t ncols, int32_t** rowptr,
+              int32_t** col, double** val, hipStream_t s)
+{
+    DevBuf<int32_t> hist, pos, bad, rp, c;
+    DevBuf<int64_t> off;
+    DevBuf<double> v;
+    RG_TRY(hist.alloc(nrows, s));
+    RG_TRY(bad.alloc(2, s));
+    RG_TRY(off.alloc(nrows + 1, s));
+    RG_TRY(pos.alloc(nnz, s));
+    RG_TRY(rp.alloc(nrows + 1, s));
+    RG_TRY(c.alloc(nnz, s));
+    RG_TRY(v.alloc(nnz, s));
+    RG_TRY(fill_zero(hist.p, nrows, s));
+    RG_TRY(fill_zero(bad.p, 2, s));
+    unsigned gs = std::min<unsigned>(grid_for(nnz, 256), 8192);
+    k_hist_i64<<<gs, 256, 0, s>>>(idx, nnz, hist.p, bad.p, nrows);
+    k_check_range<<<gs, 256, 0, s>>>(other, nnz, bad.p + 1, ncols);
+    RG_HIP(hipGetLastError());
+    int32_t nbad[2];
+    RG_TRY(read_scalars(bad.p, nbad, sizeof(nbad), s));
+    if (nbad[0] || nbad[1])
+        return fail(RG_ERR_ARGUMENT, "weight index out of range for the given "
+                                     "cell counts");
+    RG_TRY(scan_i32_to_i64(hist.p, off.p, nrows, s));
+    RG_TRY(fill_zero(hist.p, nrows, s));
+    k_group_scatter<<<gs, 256, 0, s>>>(idx, nnz, off.p, hist.p, pos.p);
+    k_group_finish<<<grid_for(nrows + 1, 128), 128, 0, s>>>(
+        off.p, nrows, pos.p, other, w, rp.p, c.p, v.p);
+    RG_HIP(hipGetLastError());
+    *rowptr = rp.release();
+    *col = c.release();
+    *val = v.release();
+    return RG_OK;
+}
+
+}  // namespace
+
+int import_triples(const int64_t* ii, const int64_t* io, const double* w,
+                   int64_t nnz, int64_t n_in, int64_t n_out, rg_weights** out,
+                   hipStream_t s)
+{
+    *out = nullptr;
+    if (nnz < 0 || n_in < 0 || n_out < 0)
+        return fail(RG_ERR_ARGUMENT, "negative size");
+    if (nnz >= INT32_MAX || n_in >= INT32_MAX || n_out >= INT32_MAX)
+        return fail(RG_ERR_ARGUMENT, "sizes beyond 2^31 - 1");
+    if (nnz > 0 && (!ii || !io || !w))
+        return fail(RG_ERR_ARGUMENT, "weight array is NULL");
+    rg_weights* W = new rg_weights();
+    W->n_in = n_in;
+    W->n_out = n_out;
+    W->nnz = nnz;
+    (void)hipGetDevice(&W->device);
+    int st = group_csr(io, ii, w, nnz, n_out, n_in, &W->rowptr_out, &W->col_out,
+                       &W->val_out, s);
+    if (st == RG_OK)
+        st = group_csr(ii, io, w, nnz, n_in, n_out, &W->rowptr_in, &W->col_in,
+                       &W->val_in, s);
+    if (st != RG_OK) {
+        weights_release(W);
+        return st;
+    }
+    *out = W;
+    return RG_OK;
+}
+
+int export_triples(const rg_weights* W, int64_t* oi, int64_t* oo, double* ov,
+                   hipStream_t s)
+{
+    if (W->nnz == 0) return RG_OK;
+    k_export<<<grid_for(W->n_in, 128), 128, 0, s>>>(W->rowptr_in, W->col_in,
+                                                     W->val_in, W->n_in, oi, oo,
+                                                     ov);
+    RG_HIP(hipGetLastError());
+    return RG_OK;
+}
+
+}  // namespace rg
+
+using namespace rg;
+
+extern "C" {
+
+int rg_build_conservative_2d(const double* xi, const double* yi, int64_t nxi,
+                             int64_t nyi, const double* xo, const double* yo,
+                             int64_t nxo, int64_t nyo, const double* wi,
+                             const rg_tuning* tuning, rg_weights_t* out,
+                             void* stream)
+{
+    if (!out) return fail(RG_ERR_ARGUMENT, "out is NULL");
+    return build_conservative_2d(xi, yi, nxi, nyi, xo, yo, nxo, nyo, wi, tuning,
+                                 out, (hipStream_t)stream);
+}
+
+int rg_weights_info(rg_weights_t w, int64_t* n_in, int64_t* n_out,
+                    int64_t* nnz)
+{
+    if (!w) return fail(RG_ERR_ARGUMENT, "weights handle is NULL");
+    if (n_in) *n_in = w->n_in;
+    if (n_out) *n_out = w->n_out;
+    if (nnz) *nnz = w->nnz;
+    return RG_OK;
+}
+
+int rg_weights_stats(rg_weights_t w, rg_build_stats* stats)
+{
+    if (!w || !stats) return fail(RG_ERR_ARGUMENT, "NULL argument");
+    *stats = w->stats;
+    return RG_OK;
+}
+
+int rg_weights_export(rg_weights_t w, int64_t* indices_input,
+                      int64_t* indices_output, double* values, void* stream)
+{
+    if (!w) return fail(RG_ERR_ARGUMENT, "weights handle is NULL");
+    if (w->nnz && (!indices_input || !indices_output || !values))
+        return fail(RG_ERR_ARGUMENT, "output array is NULL");
+    return export_triples(w, indices_input, indices_output, values,
+                          (hipStream_t)stream);
+}
+
+int rg_weights_import(const int64_t* indices_input,
+                      const int64_t* indices_output, const double* values,
+                      int64_t nnz, int64_t n_in, int64_t n_out,
+                      rg_weights_t* out, void* stream)
+{
+    if (!out) return fail(RG_ERR_ARGUMENT, "out is NULL");
+    return import_triples(indices_input, indices_output, values, nnz, n_in,
+                          n_out, out, (hipStream_t)stream);
+}
+
+int rg_weights_free(rg_weights_t w)
+{
+    weights_release(w);
+    return RG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,116 @@
+// rg_internal.hpp -- host-side plumbing shared by the HIP translation units:
+// status codes and the thread-local error string, a caching device
+// allocator, device-wide exclusive scans, and the weights object behind the
+// opaque rg_weights_t handle.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/regridding_amd.h"
+
+namespace rg {
+
+// ---- errors --------------------------------------------------------------
+
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+#define RG_HIP(expr)                                                        \
+    do {                                                                    \
+        hipError_t e_ = (expr);                                             \
+        if (e_ != hipSuccess)                                               \
+            return ::rg::fail(RG_ERR_HIP, std::string(#expr) + ": " +      \
+                                              hipGetErrorString(e_));       \
+    } while (0)
+
+#define RG_TRY(expr)                \
+    do {                            \
+        int s_ = (expr);            \
+        if (s_ != RG_OK) return s_; \
+    } while (0)
+
+// ---- device memory -------------------------------------------------------
+
+// Caching allocator: freed blocks are kept per (device, stream) and reused
+// only by later requests on the same stream, so a block released while
+// kernels that read it are still queued is never handed to concurrent work;
+// repeated builds and applies do not pay hipMalloc/hipFree.  rg_trim()
+// synchronises the device and releases the cache.
+int dev_alloc(void** p, size_t bytes, hipStream_t s);
+void dev_free(void* p);
+void dev_trim();
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { reset(); }
+    int alloc(size_t count, hipStream_t s)
+    {
+        reset();
+        n = count;
+        if (count == 0) return RG_OK;
+        return dev_alloc((void**)&p, count * sizeof(T), s);
+    }
+    void reset()
+    {
+        if (p) dev_free(p);
+        p = nullptr;
+        n = 0;
+    }
+    T* release()
+    {
+        T* q = p;
+        p = nullptr;
+        n = 0;
+        return q;
+    }
+};
+
+// Pinned host scratch for small synchronous device->host reads.
+int read_scalars(const void* dev, void* host, size_t bytes, hipStream_t s);
+
+// ---- scans ---------------------------------------------------------------
+
+// out[0..n] = exclusive prefix sum of in[0..n), out[n] = total.  `in` and
+// `out` may not alias.
+int scan_i32_to_i64(const int32_t* in, int64_t* out, int64_t n, hipStream_t s);
+int scan_i64(const int64_t* in, int64_t* out, int64_t n, hipStream_t s);
+
+// ---- fault injection (host-forced error paths for tests) -----------------
+
+bool injected_failure(const char* site);
+
+// ---- launch geometry -----------------------------------------------------
+
+inline unsigned grid_for(int64_t n, int block)
+{
+    int64_t g = (n + block - 1) / block;
+    return (unsigned)(g < 1 ? 1 : g);
+}
+
+}  // namespace rg
+
+// The object behind rg_weights_t: sparse weights of one 2-D element, stored
+// twice on the device.  CSR by input cell (columns = output cells ascending)
+// is the reference's storage order (sorted by (input, output)); CSR by
+// output cell (columns = input cells ascending) drives the forward apply.
+struct rg_weights {
+    int device = 0;
+    int64_t n_in = 0, n_out = 0, nnz = 0;
+    int32_t* rowptr_in = nullptr;   // n_in + 1
+    int32_t* col_in = nullptr;      // nnz, output cell of each entry
+    double* val_in = nullptr;       // nnz
+    int32_t* rowptr_out = nullptr;  // n_out + 1
+    int32_t* col_out = nullptr;     // nnz, input cell of each entry
+    double* val_out = nullptr;      // nnz
+    float* val_in_f32 = nullptr;    // lazily made single-precision copies
+    float* val_out_f32 = nullptr;
+    rg_build_stats stats = {};
+};

@@ -1,0 +1,408 @@
+// rg_runtime.hip -- library plumbing behind the C ABI: error strings,
+// device selection, the caching allocator, pinned scalar reads, device-wide
+// scans and the memory/stream helpers exported for ctypes hosts.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "rg_internal.hpp"
+
+namespace rg {
+
+static thread_local std::string g_error;
+
+void set_error(const std::string& msg) { g_error = msg; }
+
+int fail(int code, const std::string& msg)
+{
+    g_error = msg;
+    return code;
+}
+
+// ---- fault injection -----------------------------------------------------
+
+static std::mutex g_inject_mu;
+static std::multiset<std::string> g_inject;
+
+bool injected_failure(const char* site)
+{
+    std::lock_guard<std::mutex> lk(g_inject_mu);
+    auto it = g_inject.find(site);
+    if (it == g_inject.end()) return false;
+    g_inject.erase(it);
+    return true;
+}
+
+// ---- caching allocator ---------------------------------------------------
+
+namespace {
+
+struct BlockTag {
+    int device;
+    hipStream_t stream;
+    bool operator<(const BlockTag& o) const
+    {
+        return device != o.device ? device < o.device : stream < o.stream;
+    }
+};
+
+struct LiveBlock {
+    BlockTag tag;
+    size_t size;
+};
+
+struct Pool {
+    std::mutex mu;
+    // per (device, stream): size -> free blocks of that rounded size
+    std::map<BlockTag, std::multimap<size_t, void*>> free_blocks;
+    std::map<void*, LiveBlock> live;
+};
+
+Pool& pool()
+{
+    static Pool* p = new Pool();  // never destroyed: blocks live to exit
+    return *p;
+}
+
+size_t round_size(size_t bytes)
+{
+    if (bytes <= 512) return 512;
+    if (bytes <= (size_t)1 << 20) {  // powers of two up to 1 MiB
+        size_t r = 512;
+        while (r < bytes) r <<= 1;
+        return r;
+    }
+    const size_t g = (size_t)2 << 20;  // 2 MiB granules above
+    return (bytes + g - 1) / g * g;
+}
+
+}  // namespace
+
+int dev_alloc(void** p, size_t bytes, hipStream_t s)
+{
+    *p = nullptr;
+    if (injected_failure("alloc"))
+        return fail(RG_ERR_INJECTED, "injected allocation failure");
+    int dev = 0;
+    RG_HIP(hipGetDevice(&dev));
+    const BlockTag tag{dev, s};
+    size_t sz = round_size(bytes);
+    Pool& P = pool();
+    {
+        std::lock_guard<std::mutex> lk(P.mu);
+        auto& fb = P.free_blocks[tag];
+        auto it = fb.find(sz);
+        if (it != fb.end()) {
+            *p = it->second;
+            fb.erase(it);
+            P.live[*p] = LiveBlock{tag, sz};
+            return RG_OK;
+        }
+    }
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, sz);
+    if (e != hipSuccess) {
+        // drop the cache of this device and retry once
+        (void)hipGetLastError();
+        dev_trim();
+        e = hipMalloc(&q, sz);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(RG_ERR_MEMORY, "hipMalloc(" + std::to_string(sz) +
+                                           " bytes) failed: " +
+                                           hipGetErrorString(e));
+        }
+    }
+    std::lock_guard<std::mutex> lk(P.mu);
+    P.live[q] = LiveBlock{tag, sz};
+    *p = q;
+    return RG_OK;
+}
+
+void dev_free(void* p)
+{
+    if (!p) return;
+    Pool& P = pool();
+    std::lock_guard<std::mutex> lk(P.mu);
+    auto it = P.live.find(p);
+    if (it == P.live.end()) return;
+    P.free_blocks[it->second.tag].emplace(it->second.size, p);
+    P.live.erase(it);
+}
+
+void dev_trim()
+{
+    Pool& P = pool();
+    std::lock_guard<std::mutex> lk(P.mu);
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    for (auto& kv : P.free_blocks) {
+        (void)hipSetDevice(kv.first.device);
+        // blocks may still be read by work queued before their release
+        (void)hipDeviceSynchronize();
+        for (auto& b : kv.second) (void)hipFree(b.second);
+        kv.second.clear();
+    }
+    (void)hipSetDevice(cur);
+}
+
+// ---- pinned scalar reads -------------------------------------------------
+
+int read_scalars(const void* dev, void* host, size_t bytes, hipStream_t s)
+{
+    static thread_local void* pinned = nullptr;
+    static thread_local size_t pinned_size = 0;
+    if (bytes > pinned_size) {
+        if (pinned) (void)hipHostFree(pinned);
+        pinned = nullptr;
+        size_t sz = bytes < 4096 ? 4096 : bytes;
+        RG_HIP(hipHostMalloc(&pinned, sz, hipHostMallocDefault));
+        pinned_size = sz;
+    }
+    RG_HIP(hipMemcpyAsync(pinned, dev, bytes, hipMemcpyDeviceToHost, s));
+    RG_HIP(hipStreamSynchronize(s));
+    std::memcpy(host, pinned, bytes);
+    return RG_OK;
+}
+
+// ---- scans ---------------------------------------------------------------
+
+namespace {
+
+constexpr int SCAN_BLOCK = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
+
+// Block-wide exclusive scan of one value per thread (wavefront shuffles,
+// then the 4 wave totals through LDS).
+__device__ int64_t block_exclusive(int64_t v, int64_t* total)
+{
+    __shared__ int64_t wsum[SCAN_BLOCK / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int64_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int64_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wid] = x;
+    __syncthreads();
+    int64_t base = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < SCAN_BLOCK / 64; w++) {
+        if (w < wid) base += wsum[w];
+        all += wsum[w];
+    }
+    __syncthreads();
+    *total = all;
+    return base + x - v;
+}
+
+template <class T>
+__global__ void k_scan_tiles(const T* __restrict__ in, int64_t* __restrict__ out,
+                             int64_t* __restrict__ tile_sums, int64_t n)
+{
+    const int64_t tile0 = (int64_t)blockIdx.x * SCAN_TILE;
+    const int64_t base = tile0 + (int64_t)threadIdx.x * SCAN_ITEMS;
+    int64_t v[SCAN_ITEMS];
+    int64_t s = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; q++) {
+        int64_t i = base + q;
+        v[q] = i < n ? (int64_t)in[i] : 0;
+        s += v[q];
+    }
+    int64_t total;
+    int64_t run = block_exclusive(s, &total);
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; q++) {
+        int64_t i = base + q;
+        if (i < n) out[i] = run;
+        run += v[q];
+    }
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+__global__ void k_scan_add(int64_t* __restrict__ out,
+                           const int64_t* __restrict__ tile_off, int64_t n)
+{
+    const int64_t tile0 = (int64_t)blockIdx.x * SCAN_TILE;
+    const int64_t add = tile_off[blockIdx.x];
+    for (int q = threadIdx.x; q < SCAN_TILE; q += SCAN_BLOCK) {
+        int64_t i = tile0 + q;
+        if (i < n) out[i] += add;
+    }
+}
+
+__global__ void k_set_total(int64_t* out, int64_t n, const int64_t* tile_off,
+                            int64_t ntiles)
+{
+    out[n] = tile_off[ntiles];
+}
+
+__global__ void k_copy_total(int64_t* out, int64_t n, const int64_t* tile_sums)
+{
+    out[n] = tile_sums[0];
+}
+
+template <class T>
+int scan_impl(const T* in, int64_t* out, int64_t n, hipStream_t s)
+{
+    if (n == 0) {
+        RG_HIP(hipMemsetAsync(out, 0, sizeof(int64_t), s));
+        return RG_OK;
+    }
+    int64_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    DevBuf<int64_t> sums, offs;
+    RG_TRY(sums.alloc(ntiles, s));
+    RG_TRY(offs.alloc(ntiles + 1, s));
+    k_scan_tiles<T><<<(unsigned)ntiles, SCAN_BLOCK, 0, s>>>(in, out, sums.p, n);
+    RG_HIP(hipGetLastError());
+    if (ntiles == 1) {  // one tile: its sum is the total, nothing to add
+        k_copy_total<<<1, 1, 0, s>>>(out, n, sums.p);
+        RG_HIP(hipGetLastError());
+        return RG_OK;
+    }
+    RG_TRY(scan_impl<int64_t>(sums.p, offs.p, ntiles, s));
+    k_scan_add<<<(unsigned)ntiles, SCAN_BLOCK, 0, s>>>(out, offs.p, n);
+    RG_HIP(hipGetLastError());
+    k_set_total<<<1, 1, 0, s>>>(out, n, offs.p, ntiles);
+    RG_HIP(hipGetLastError());
+    return RG_OK;
+}
+
+}  // namespace
+
+int scan_i32_to_i64(const int32_t* in, int64_t* out, int64_t n, hipStream_t s)
+{
+    return scan_impl<int32_t>(in, out, n, s);
+}
+
+int scan_i64(const int64_t* in, int64_t* out, int64_t n, hipStream_t s)
+{
+    return scan_impl<int64_t>(in, out, n, s);
+}
+
+}  // namespace rg
+
+// ---- exported plumbing ---------------------------------------------------
+
+using namespace rg;
+
+extern "C" {
+
+const char* rg_last_error(void) { return g_error.c_str(); }
+
+int rg_version(void) { return 1; }
+
+int rg_init(int device)
+{
+    int n = 0;
+    RG_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n)
+        return fail(RG_ERR_ARGUMENT, "device " + std::to_string(device) +
+                                         " out of range (" +
+                                         std::to_string(n) + " devices)");
+    RG_HIP(hipSetDevice(device));
+    RG_HIP(hipFree(nullptr));  // create the context now
+    return RG_OK;
+}
+
+int rg_device_count(int* count)
+{
+    if (!count) return fail(RG_ERR_ARGUMENT, "count is NULL");
+    RG_HIP(hipGetDeviceCount(count));
+    return RG_OK;
+}
+
+int rg_mem_info(size_t* free_bytes, size_t* total_bytes)
+{
+    size_t f = 0, t = 0;
+    RG_HIP(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return RG_OK;
+}
+
+void rg_trim(void) { dev_trim(); }
+
+int rg_malloc(void** ptr, size_t bytes)
+{
+    if (!ptr) return fail(RG_ERR_ARGUMENT, "ptr is NULL");
+    return dev_alloc(ptr, bytes, nullptr);
+}
+
+int rg_free(void* ptr)
+{
+    dev_free(ptr);
+    return RG_OK;
+}
+
+int rg_memcpy_htod(void* dst, const void* src, size_t bytes, void* stream)
+{
+    if (bytes == 0) return RG_OK;
+    RG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice,
+                          (hipStream_t)stream));
+    return RG_OK;
+}
+
+int rg_memcpy_dtoh(void* dst, const void* src, size_t bytes, void* stream)
+{
+    if (bytes == 0) return RG_OK;
+    RG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost,
+                          (hipStream_t)stream));
+    RG_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return RG_OK;
+}
+
+int rg_memset(void* dst, int value, size_t bytes, void* stream)
+{
+    if (bytes == 0) return RG_OK;
+    RG_HIP(hipMemsetAsync(dst, value, bytes, (hipStream_t)stream));
+    return RG_OK;
+}
+
+int rg_stream_create(void** stream)
+{
+    if (!stream) return fail(RG_ERR_ARGUMENT, "stream is NULL");
+    hipStream_t s = nullptr;
+    RG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = (void*)s;
+    return RG_OK;
+}
+
+int rg_stream_destroy(void* stream)
+{
+    if (stream) RG_HIP(hipStreamDestroy((hipStream_t)stream));
+    return RG_OK;
+}
+
+int rg_stream_sync(void* stream)
+{
+    RG_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return RG_OK;
+}
+
+int rg_device_sync(void)
+{
+    RG_HIP(hipDeviceSynchronize());
+    return RG_OK;
+}
+
+int rg_inject_failure(const char* site)
+{
+    std::lock_guard<std::mutex> lk(g_inject_mu);
+    if (!site) {
+        g_inject.clear();
+        return RG_OK;
+    }
+    g_inject.insert(site);
+    return RG_OK;
+}
+
+}  // extern "C"

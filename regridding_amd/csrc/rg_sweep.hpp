@@ -1,0 +1,546 @@
+// rg_sweep.hpp -- the Ramshaw edge sweep, decomposed into independent chunks.
+//
+// The reference walks each row of the sweep grid serially
+// (_weights_conservative_2d.py:286-387).  At every sweep VERTEX the walk state
+// collapses to "outside the static grid" or "inside static cell c", with the
+// last-crossed-edge memory reset (the no-hit branches at :540 and :706-710),
+// and the walk position is exactly the vertex.  A row can therefore be cut at
+// any vertex: the piece that starts there only needs that collapsed state.
+//
+// The GPU build gives each lane a chunk of K consecutive edges of one row.
+// Chunk 0 of every row starts from the reference's own row-start state
+// (bounding box, boundary polygon and secant search, :308-322).  Later chunks
+// start from a guess (the secant cell search at their first vertex).  After
+// all chunks have walked, a per-row pass compares every chunk's start state
+// with the end state of its predecessor; a chunk whose guess was wrong is
+// walked again from the true state, so the result is exactly the serial walk
+// whatever the guesses were.
+//
+// All functions are __host__ __device__: tests/emu runs them on the CPU
+// against the oracle before any GPU launch.
+#pragma once
+
+#include "rg_geometry.hpp"
+
+namespace rg {
+
+// Boundary-edge blocking of the outside-step scan.  Every edge's bounding box
+// lies inside its block's box, and every block's box inside its super-block's
+// box, so skipping a box the segment's bounding box does not touch skips
+// only edges for which the reference's first test (bounding boxes,
+// geometry.py:422-433) already fails.
+constexpr int BND_BLOCK = 16;
+constexpr int BND_SUPER = 16;  // blocks per super-block
+
+struct Box {
+    double x0, y0, x1, y1;  // inclusive [x0, x1] x [y0, y1]
+};
+
+// A static (searched) grid with its derived tables, all in device memory.
+struct StaticGrid {
+    const double* x;
+    const double* y;
+    int64_t nx, ny;          // vertices
+    const double* bbox;      // 4 values: min x, min y, max x, max y (numpy
+                             // semantics: NaN propagates)
+    const double* bpx;       // boundary polygon, grid_boundary order
+    const double* bpy;
+    int64_t nbp;
+    const double* be;        // boundary edges in scan order, 4 doubles each
+    const int32_t* bcell;    // flat static cell of each boundary edge
+    int64_t nbe;
+    const Box* bblk;         // box per BND_BLOCK edges
+    int64_t nblk;
+    const Box* bsup;         // box per BND_SUPER blocks
+    int64_t nsup;
+};
+
+// One (pass, axis) sweep: rows of the aligned sweep grid walked across the
+// static grid (_sweep_grid, :133-203).
+struct Pass {
+    const double* xs;        // sweep grid vertices, original orientation
+    const double* ys;
+    int64_t sr, sk;          // element (row, k) of the aligned view
+    int64_t nrows, nk;       // aligned view shape
+    int64_t ncy_sweep;       // cells along the last axis of the sweep grid
+    int32_t axis;            // 0: aligned view is the transpose
+    int32_t sweep_input;     // 1: sweep = input grid, static = output grid
+    StaticGrid st;
+    int64_t chunk_base;      // global id of the first chunk of this pass
+    int64_t cpr;             // chunks per row
+};
+
+struct Build {
+    Pass pass[4];            // F axis 0, F axis 1, T axis 0, T axis 1
+    const double* vol_in;    // input cell areas, flat
+    const double* wi;        // optional input weights, flat (or null)
+    int64_t K;               // edges per chunk
+    int64_t nchunks;
+    int64_t nrows_total;
+    int64_t row_base[4];     // global id of the first row of each pass
+};
+
+// Per-step budget of a chunk: a non-degenerate walk moves at most one cell
+// per step and crosses fewer than 2 (nx + ny) cells per edge; the reference
+// walk can cycle forever only on exactly degenerate corners.  A chunk that
+// exceeds the budget fails the build instead of hanging the GPU.
+RG_HD int64_t step_limit(const StaticGrid& s, int64_t nedges)
+{
+    return nedges * (2 * (s.nx + s.ny) + 8) + 64;
+}
+
+// Area of cell (a, b) of an (nx, ny) vertex grid.  The reference starts
+// from 0 and adds, for the axis-0 sweep (x and y swapped on the transposed
+// view), -T(a, b) then +T(a, b+1), then for the axis-1 sweep -S(a, b) then
+// +S(a+1, b) (_grids.py:84-140); this is that sequence for one cell.
+RG_HD double cell_area(const double* __restrict__ x,
+                       const double* __restrict__ y, int64_t ny, int64_t a,
+                       int64_t b)
+{
+    const int64_t v00 = a * ny + b, v10 = v00 + ny;
+    double acc = 0.0;
+    acc -= area_tri(y[v00], x[v00], y[v10], x[v10]);
+    acc += area_tri(y[v00 + 1], x[v00 + 1], y[v10 + 1], x[v10 + 1]);
+    acc -= area_tri(x[v00], y[v00], x[v00 + 1], y[v00 + 1]);
+    acc += area_tri(x[v10], y[v10], x[v10 + 1], y[v10 + 1]);
+    return acc;
+}
+
+RG_HD int pass_of_chunk(const Build& B, int64_t g)
+{
+    int p = 0;
+    if (g >= B.pass[1].chunk_base) p = 1;
+    if (g >= B.pass[2].chunk_base) p = 2;
+    if (g >= B.pass[3].chunk_base) p = 3;
+    return p;
+}
+
+RG_HD int pass_of_row(const Build& B, int64_t r)
+{
+    int p = 0;
+    if (r >= B.row_base[1]) p = 1;
+    if (r >= B.row_base[2]) p = 2;
+    if (r >= B.row_base[3]) p = 3;
+    return p;
+}
+
+RG_HD double sweep_x(const Pass& P, int64_t row, int64_t k)
+{
+    return P.xs[row * P.sr + k * P.sk];
+}
+
+RG_HD double sweep_y(const Pass& P, int64_t row, int64_t k)
+{
+    return P.ys[row * P.sr + k * P.sk];
+}
+
+// Scan-order code of boundary edge e: 2 * face + axis, the value the
+// reference stores in index_edge_last after entering (:532-535).
+RG_HD int boundary_code(const StaticGrid& s, int64_t e)
+{
+    int64_t na = s.ny - 1;
+    int64_t nb = s.nx - 1;
+    if (e < na) return 0;          // axis 0, face 0 (i = 0)
+    if (e < 2 * na) return 2;      // axis 0, face 1 (i = last)
+    if (e < 2 * na + nb) return 1; // axis 1, face 0 (j = 0)
+    return 3;                      // axis 1, face 1 (j = last)
+}
+
+// Boundary edge e of the outside scan (:462-537): its end points (q1, q2) as
+// the reference orders them and its static cell.  Used to fill the tables.
+RG_HD void boundary_edge(const double* x, const double* y, int64_t nx,
+                         int64_t ny, int64_t e, double* q, int32_t* cell)
+{
+    int64_t na = ny - 1, nb = nx - 1;
+    int64_t ncy = ny - 1;
+    int64_t v1, v2, c;
+    if (e < na) {                   // vertices (0, i0) -> (0, i0 + 1)
+        int64_t i0 = e;
+        v1 = i0; v2 = i0 + 1; c = i0;
+    } else if (e < 2 * na) {        // (nx-1, i0) -> (nx-1, i0 + 1)
+        int64_t i0 = e - na;
+        v1 = (nx - 1) * ny + i0; v2 = v1 + 1; c = (nx - 2) * ncy + i0;
+    } else if (e < 2 * na + nb) {   // (i0, 0) -> (i0 + 1, 0)
+        int64_t i0 = e - 2 * na;
+        v1 = i0 * ny; v2 = (i0 + 1) * ny; c = i0 * ncy;
+    } else {                        // (i0, ny-1) -> (i0 + 1, ny-1)
+        int64_t i0 = e - 2 * na - nb;
+        v1 = i0 * ny + ny - 1; v2 = (i0 + 1) * ny + ny - 1; c = i0 * ncy + ncy - 1;
+    }
+    q[0] = x[v1]; q[1] = y[v1]; q[2] = x[v2]; q[3] = y[v2];
+    *cell = (int32_t)c;
+}
+
+// Vertex k of the boundary polygon (_grids.py:167-215 order).
+RG_HD int64_t boundary_vertex(int64_t nx, int64_t ny, int64_t k)
+{
+    int64_t a = nx - 1, b = ny - 1;
+    if (k < a) return k * ny;                          // j = 0, i up
+    k -= a;
+    if (k < b) return (nx - 1) * ny + k;               // i = last, j up
+    k -= b;
+    if (k < a) return (nx - 1 - k) * ny + (ny - 1);    // j = last, i down
+    k -= a;
+    return (ny - 1 - k);                               // i = 0, j down
+}
+
+RG_HD bool box_touch(const Box& b, double xl, double yl, double xh, double yh)
+{
+    return xl <= b.x1 && b.x0 <= xh && yl <= b.y1 && b.y0 <= yh;
+}
+
+// _step_outside_static (:401-552): the earliest boundary crossing of
+// p1 -> p2, ties resolved by scan order, ignoring the cell just exited.
+// Returns true and (cell, code, t) on a crossing.
+RG_HD bool outside_scan(const StaticGrid& s, double x1, double y1, double x2,
+                        double y2, int64_t skip_cell, int64_t* cell, int* code,
+                        double* t_hit)
+{
+    double xl = x1 > x2 ? x2 : x1, xh = x1 > x2 ? x1 : x2;
+    double yl = y1 > y2 ? y2 : y1, yh = y1 > y2 ? y1 : y2;
+    double t_min = __builtin_huge_val();
+    bool found = false;
+    int64_t best = -1;
+    for (int64_t su = 0; su < s.nsup; su++) {
+        if (!box_touch(s.bsup[su], xl, yl, xh, yh)) continue;
+        int64_t b1 = (su + 1) * BND_SUPER;
+        if (b1 > s.nblk) b1 = s.nblk;
+        for (int64_t bl = su * BND_SUPER; bl < b1; bl++) {
+            if (!box_touch(s.bblk[bl], xl, yl, xh, yh)) continue;
+            int64_t e1 = (bl + 1) * BND_BLOCK;
+            if (e1 > s.nbe) e1 = s.nbe;
+            for (int64_t e = bl * BND_BLOCK; e < e1; e++) {
+                const double* q = s.be + 4 * e;
+                TU r = seg_params(x1, y1, x2, y2, q[0], q[1], q[2], q[3]);
+                if (seg_hit(r) && r.t < t_min) {
+                    if ((int64_t)s.bcell[e] == skip_cell) continue;
+                    found = true;
+                    t_min = r.t;
+                    best = e;
+                }
+            }
+        }
+    }
+    if (found) {
+        *cell = s.bcell[best];
+        *code = boundary_code(s, best);
+        *t_hit = t_min;
+    }
+    return found;
+}
+
+// Walk end/start states: -1 outside, otherwise the flat static cell.
+constexpr int32_t STATE_OUTSIDE = -1;
+constexpr int32_t STATE_FAILED = -2;   // step budget exceeded
+
+// Walk edges [k0, k1) of `row` from `start`, calling sink.emit(in, out, w)
+// for every triple in the reference's order (left then right per
+// sub-segment, :786-830).  Returns the state at vertex k1.
+template <class Sink>
+RG_HD int32_t walk_chunk(const Build& B, const Pass& P, int64_t row,
+                         int64_t k0, int64_t k1, int32_t start, Sink& sink)
+{
+    const StaticGrid& s = P.st;
+    const int64_t ncx = s.nx - 1, ncy = s.ny - 1;
+    const int64_t limit = step_limit(s, k1 - k0);
+    double x1 = sweep_x(P, row, k0), y1 = sweep_y(P, row, k0);
+    int64_t k = k0;
+    bool outside = start < 0;
+    int64_t sx = outside ? 0 : (int64_t)start / ncy;
+    int64_t sy = outside ? 0 : (int64_t)start % ncy;
+    int64_t edge_last = -1;  // sys.maxsize
+    int64_t steps = 0;
+    while (k < k1) {
+        if (++steps > limit) return STATE_FAILED;
+        double x2 = sweep_x(P, row, k + 1), y2 = sweep_y(P, row, k + 1);
+        if (outside) {
+            int64_t cell;
+            int code;
+            double t;
+            if (outside_scan(s, x1, y1, x2, y2, edge_last, &cell, &code, &t)) {
+                x2 = x1 + t * (x2 - x1);
+                y2 = y1 + t * (y2 - y1);
+                sx = cell / ncy;
+                sy = cell % ncy;
+                edge_last = code;
+                outside = false;
+            } else {
+                k += 1;
+                edge_last = -1;
+            }
+        } else {
+            // _step_inside_static (:561-749).  Cell corners c0..c3 =
+            // (0,0) (1,0) (1,1) (0,1); edge v runs c[v-1] -> c[v] and its
+            // outward neighbour is cell_normals[v].
+            const int64_t a = sx * s.ny + sy, b = (sx + 1) * s.ny + sy;
+            const double cx0 = s.x[a], cy0 = s.y[a];
+            const double cx1 = s.x[b], cy1 = s.y[b];
+            const double cx2 = s.x[b + 1], cy2 = s.y[b + 1];
+            const double cx3 = s.x[a + 1], cy3 = s.y[a + 1];
+            int hit = -1;
+            TU r;
+            r.t = 0;
+            if (edge_last != 0) {
+                r = seg_params(x1, y1, x2, y2, cx3, cy3, cx0, cy0);
+                if (seg_hit(r)) hit = 0;
+            }
+            if (hit < 0 && edge_last != 1) {
+                r = seg_params(x1, y1, x2, y2, cx0, cy0, cx1, cy1);
+                if (seg_hit(r)) hit = 1;
+            }
+            if (hit < 0 && edge_last != 2) {
+                r = seg_params(x1, y1, x2, y2, cx1, cy1, cx2, cy2);
+                if (seg_hit(r)) hit = 2;
+            }
+            if (hit < 0 && edge_last != 3) {
+                r = seg_params(x1, y1, x2, y2, cx2, cy2, cx3, cy3);
+                if (seg_hit(r)) hit = 3;
+            }
+            int64_t nsx = sx, nsy = sy, k_next = k;
+            int64_t e_next;
+            if (hit >= 0) {
+                x2 = x1 + r.t * (x2 - x1);
+                y2 = y1 + r.t * (y2 - y1);
+                nsx += (hit == 0) ? -1 : (hit == 2 ? 1 : 0);
+                nsy += (hit == 1) ? -1 : (hit == 3 ? 1 : 0);
+                e_next = (hit + 2) & 3;
+            } else {
+                k_next = k + 1;
+                e_next = -1;
+            }
+            // _calc_and_save_weights (:757-830) for p1 -> p2 at sweep
+            // index (row, k) and static cell (sx, sy).
+            double area = area_tri(x1, y1, x2, y2);
+            if (P.axis == 0) area = -area;
+            const int64_t cell_static = sx * ncy + sy;
+            for (int side = 0; side < 2; side++) {
+                int64_t rr = side == 0 ? row - 1 : row;
+                if (side == 0 && !(rr >= 0)) continue;
+                if (side == 1 && !(rr < P.nrows - 1)) continue;
+                int64_t cell_sweep =
+                    P.axis == 0 ? k * P.ncy_sweep + rr : rr * P.ncy_sweep + k;
+                int64_t cin = P.sweep_input ? cell_sweep : cell_static;
+                int64_t cout = P.sweep_input ? cell_static : cell_sweep;
+                double vol = B.vol_in[cin];
+                double w = side == 0 ? area / vol : -area / vol;
+                if (B.wi) w *= B.wi[cin];
+                sink.emit(cin, cout, w);
+            }
+            if (!(nsx >= 0 && nsy >= 0 && nsx < ncx && nsy < ncy)) {
+                e_next = cell_static;
+                outside = true;
+            } else {
+                sx = nsx;
+                sy = nsy;
+            }
+            k = k_next;
+            edge_last = e_next;
+        }
+        x1 = x2;
+        y1 = y2;
+    }
+    return outside ? STATE_OUTSIDE : (int32_t)(sx * ncy + sy);
+}
+
+// Guessed walk state at the first vertex of chunk c > 0 of a row: the
+// reference's own cell search (bounding box, then secant iteration), with
+// "outside" for anything it does not place in a cell.  Only a guess: the
+// per-row consistency pass corrects it.
+RG_HD int32_t chunk_start_guess(const Pass& P, int64_t row, int64_t k)
+{
+    const StaticGrid& s = P.st;
+    double px = sweep_x(P, row, k), py = sweep_y(P, row, k);
+    if (!(s.bbox[0] <= px && px <= s.bbox[2] && s.bbox[1] <= py &&
+          py <= s.bbox[3]))
+        return STATE_OUTSIDE;
+    int64_t c = index_secant(s.x, s.y, s.nx, s.ny, px, py);
+    return c >= 0 ? (int32_t)c : STATE_OUTSIDE;
+}
+
+// numpy's pairwise summation of a[0..n) (the float64 add-reduce inner loop
+// used by np.add.reduceat): under 8 terms a sequential sum seeded with -0.0,
+// up to 128 terms eight interleaved accumulators, above that a split at n/2
+// rounded down to a multiple of 8.  The split recursion is evaluated with an
+// explicit stack (depth <= 64 for any int64 n).
+RG_HD double pairwise_leaf(const double* a, int64_t n)
+{
+    if (n < 8) {
+        double res = -0.0;
+        for (int64_t i = 0; i < n; i++) res += a[i];
+        return res;
+    }
+    double r0 = a[0], r1 = a[1], r2 = a[2], r3 = a[3];
+    double r4 = a[4], r5 = a[5], r6 = a[6], r7 = a[7];
+    int64_t i;
+    for (i = 8; i < n - (n % 8); i += 8) {
+        r0 += a[i + 0]; r1 += a[i + 1]; r2 += a[i + 2]; r3 += a[i + 3];
+        r4 += a[i + 4]; r5 += a[i + 5]; r6 += a[i + 6]; r7 += a[i + 7];
+    }
+    double res = ((r0 + r1) + (r2 + r3)) + ((r4 + r5) + (r6 + r7));
+    for (; i < n; i++) res += a[i];
+    return res;
+}
+
+RG_HD double pairwise_sum(const double* a, int64_t n)
+{
+    if (n <= 128) return pairwise_leaf(a, n);
+    // post-order traversal of the split tree
+    struct Frame {
+        int64_t off, n;
+        int stage;
+        double left;
+    };
+    Frame st[64];
+    int top = 0;
+    st[0].off = 0; st[0].n = n; st[0].stage = 0; st[0].left = 0;
+    double ret = 0;
+    while (top >= 0) {
+        Frame& f = st[top];
+        if (f.n <= 128) {
+            ret = pairwise_leaf(a + f.off, f.n);
+            top--;
+            continue;
+        }
+        int64_t n2 = f.n / 2;
+        n2 -= n2 % 8;
+        if (f.stage == 0) {
+            f.stage = 1;
+            Frame& c = st[top + 1];
+            c.off = f.off; c.n = n2; c.stage = 0; c.left = 0;
+            top++;
+        } else if (f.stage == 1) {
+            f.left = ret;
+            f.stage = 2;
+            Frame& c = st[top + 1];
+            c.off = f.off + n2; c.n = f.n - n2; c.stage = 0; c.left = 0;
+            top++;
+        } else {
+            ret = f.left + ret;
+            top--;
+        }
+    }
+    return ret;
+}
+
+// np.add.reduceat over one segment: first element plus the pairwise sum of
+// the rest (_weights_arrays.py:72).
+RG_HD double reduceat_segment(const double* a, int64_t n)
+{
+    if (n == 1) return a[0];
+    return a[0] + pairwise_sum(a + 1, n - 1);
+}
+
+// Sort keys[0..n) ascending with vals following.  Keys are unique, so the
+// result is fully determined.  Insertion sort for short ranges, heapsort
+// beyond (bounded work, no extra memory).
+template <class K, class V>
+RG_HD void sort_pairs(K* keys, V* vals, int64_t n)
+{
+    if (n <= 48) {
+        for (int64_t i = 1; i < n; i++) {
+            K k = keys[i];
+            V v = vals[i];
+            int64_t j = i - 1;
+            while (j >= 0 && keys[j] > k) {
+                keys[j + 1] = keys[j];
+                vals[j + 1] = vals[j];
+                j--;
+            }
+            keys[j + 1] = k;
+            vals[j + 1] = v;
+        }
+        return;
+    }
+    // heapsort
+    for (int64_t start = n / 2 - 1; start >= 0; start--) {
+        int64_t root = start;
+        while (2 * root + 1 < n) {
+            int64_t child = 2 * root + 1;
+            if (child + 1 < n && keys[child] < keys[child + 1]) child++;
+            if (keys[root] < keys[child]) {
+                K tk = keys[root]; keys[root] = keys[child]; keys[child] = tk;
+                V tv = vals[root]; vals[root] = vals[child]; vals[child] = tv;
+                root = child;
+            } else {
+                break;
+            }
+        }
+    }
+    for (int64_t end = n - 1; end > 0; end--) {
+        K tk = keys[0]; keys[0] = keys[end]; keys[end] = tk;
+        V tv = vals[0]; vals[0] = vals[end]; vals[end] = tv;
+        int64_t root = 0;
+        while (2 * root + 1 < end) {
+            int64_t child = 2 * root + 1;
+            if (child + 1 < end && keys[child] < keys[child + 1]) child++;
+            if (keys[root] < keys[child]) {
+                K tk2 = keys[root]; keys[root] = keys[child]; keys[child] = tk2;
+                V tv2 = vals[root]; vals[root] = vals[child]; vals[child] = tv2;
+                root = child;
+            } else {
+                break;
+            }
+        }
+    }
+}
+
+// Coalesce one input cell's bucket: keys are (out << 32 | ordinal), values
+// the emitted weights.  After sorting, runs of equal `out` are the duplicate
+// (in, out) pairs in emission order; each run is reduced in reduceat order
+// (_weights_arrays.py:59-72).  The unique (out, weight) pairs are written
+// back to the front of the bucket; returns their count.
+//
+// A run of more than 129 terms needs the recursive split of the pairwise sum
+// (with a stack in scratch memory).  Such runs do not occur for cells that
+// overlap in a polygon of a few edges; if one does, the bucket is left
+// sorted but not compacted, *long_run is set, and bucket_reduce_long (run by
+// a separate kernel) finishes it.
+RG_HD int64_t bucket_reduce(uint64_t* keys, double* vals, int64_t n,
+                            bool* long_run)
+{
+    sort_pairs(keys, vals, n);
+    int64_t m = 0, p = 0;
+    bool lr = false;
+    while (p < n) {
+        uint64_t out = keys[p] >> 32;
+        int64_t q = p + 1;
+        while (q < n && (keys[q] >> 32) == out) q++;
+        if (q - p - 1 > 128) lr = true;
+        m++;
+        p = q;
+    }
+    *long_run = lr;
+    if (lr) return m;
+    m = 0;
+    p = 0;
+    while (p < n) {
+        uint64_t out = keys[p] >> 32;
+        int64_t q = p + 1;
+        while (q < n && (keys[q] >> 32) == out) q++;
+        double v = q - p == 1 ? vals[p] : vals[p] + pairwise_leaf(vals + p + 1, q - p - 1);
+        keys[m] = out;
+        vals[m] = v;
+        m++;
+        p = q;
+    }
+    return m;
+}
+
+// Completion of a bucket left sorted by bucket_reduce (any run length).
+RG_HD int64_t bucket_reduce_long(uint64_t* keys, double* vals, int64_t n)
+{
+    int64_t m = 0, p = 0;
+    while (p < n) {
+        uint64_t out = keys[p] >> 32;
+        int64_t q = p + 1;
+        while (q < n && (keys[q] >> 32) == out) q++;
+        double v = reduceat_segment(vals + p, q - p);
+        keys[m] = out;
+        vals[m] = v;
+        m++;
+        p = q;
+    }
+    return m;
+}
+
+}  // namespace rg

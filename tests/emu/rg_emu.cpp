@@ -1,0 +1,375 @@
+// rg_emu.cpp -- CPU emulation of the GPU build (test infrastructure).
+//
+// Runs the per-lane bodies of the GPU kernels (regridding_amd/csrc/
+// rg_sweep.hpp: row start, chunk start guess, chunked walk, consistency
+// fix-up, bucket reduction) serially on the host, with the same chunk
+// decomposition and order of phases as rg_build.hip.  The tests compare its
+// emission sequence and coalesced weights with the oracle bit for bit, which
+// checks the chunk decomposition and every device function without a GPU.
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../regridding_amd/csrc/rg_sweep.hpp"
+
+using namespace rg;
+
+namespace {
+
+struct HCount {
+    int32_t n;
+    int32_t* hist;
+    __host__ __device__ void emit(int64_t cin, int64_t, double)
+    {
+        n++;
+        if (hist) hist[cin]++;
+    }
+};
+
+struct HUncount {
+    int32_t* hist;
+    __host__ __device__ void emit(int64_t cin, int64_t, double) { hist[cin]--; }
+};
+
+struct HFlat {
+    int64_t pos;
+    int64_t* in;
+    int64_t* out;
+    double* w;
+    __host__ __device__ void emit(int64_t cin, int64_t cout, double v)
+    {
+        in[pos] = cin;
+        out[pos] = cout;
+        w[pos] = v;
+        pos++;
+    }
+};
+
+struct HBucket {
+    uint32_t ordinal;
+    const int64_t* off;
+    int32_t* cursor;
+    uint64_t* key;
+    double* val;
+    __host__ __device__ void emit(int64_t cin, int64_t cout, double v)
+    {
+        int64_t p = off[cin] + cursor[cin]++;
+        key[p] = ((uint64_t)cout << 32) | ordinal;
+        val[p] = v;
+        ordinal++;
+    }
+};
+
+double nmin(double a, double b)
+{
+    return (a != a || b != b) ? __builtin_nan("") : (a < b ? a : b);
+}
+double nmax(double a, double b)
+{
+    return (a != a || b != b) ? __builtin_nan("") : (a > b ? a : b);
+}
+
+struct Tables {
+    std::vector<double> bbox, bpx, bpy, be;
+    std::vector<int32_t> bcell;
+    std::vector<Box> bblk, bsup;
+};
+
+void prepare(const double* x, const double* y, int64_t nx, int64_t ny,
+             Tables& T, StaticGrid& G)
+{
+    int64_t nv = nx * ny, nbe = 2 * (nx - 1) + 2 * (ny - 1);
+    T.bbox = {x[0], y[0], x[0], y[0]};
+    for (int64_t i = 0; i < nv; i++) {
+        T.bbox[0] = nmin(T.bbox[0], x[i]);
+        T.bbox[1] = nmin(T.bbox[1], y[i]);
+        T.bbox[2] = nmax(T.bbox[2], x[i]);
+        T.bbox[3] = nmax(T.bbox[3], y[i]);
+    }
+    T.bpx.resize(nbe);
+    T.bpy.resize(nbe);
+    T.be.resize(4 * nbe);
+    T.bcell.resize(nbe);
+    for (int64_t e = 0; e < nbe; e++) {
+        int64_t v = boundary_vertex(nx, ny, e);
+        T.bpx[e] = x[v];
+        T.bpy[e] = y[v];
+        boundary_edge(x, y, nx, ny, e, &T.be[4 * e], &T.bcell[e]);
+    }
+    int64_t nblk = (nbe + BND_BLOCK - 1) / BND_BLOCK;
+    int64_t nsup = (nblk + BND_SUPER - 1) / BND_SUPER;
+    T.bblk.resize(nblk);
+    for (int64_t b = 0; b < nblk; b++) {
+        Box r{__builtin_huge_val(), __builtin_huge_val(), -__builtin_huge_val(),
+              -__builtin_huge_val()};
+        for (int64_t e = b * BND_BLOCK; e < std::min(nbe, (b + 1) * BND_BLOCK); e++) {
+            const double* q = &T.be[4 * e];
+            r.x0 = fmin(r.x0, fmin(q[0], q[2]));
+            r.x1 = fmax(r.x1, fmax(q[0], q[2]));
+            r.y0 = fmin(r.y0, fmin(q[1], q[3]));
+            r.y1 = fmax(r.y1, fmax(q[1], q[3]));
+        }
+        T.bblk[b] = r;
+    }
+    T.bsup.resize(nsup);
+    for (int64_t s = 0; s < nsup; s++) {
+        Box r{__builtin_huge_val(), __builtin_huge_val(), -__builtin_huge_val(),
+              -__builtin_huge_val()};
+        for (int64_t b = s * BND_SUPER; b < std::min(nblk, (s + 1) * BND_SUPER); b++) {
+            r.x0 = fmin(r.x0, T.bblk[b].x0);
+            r.y0 = fmin(r.y0, T.bblk[b].y0);
+            r.x1 = fmax(r.x1, T.bblk[b].x1);
+            r.y1 = fmax(r.y1, T.bblk[b].y1);
+        }
+        T.bsup[s] = r;
+    }
+    G.x = x; G.y = y; G.nx = nx; G.ny = ny;
+    G.bbox = T.bbox.data();
+    G.bpx = T.bpx.data(); G.bpy = T.bpy.data(); G.nbp = nbe;
+    G.be = T.be.data(); G.bcell = T.bcell.data(); G.nbe = nbe;
+    G.bblk = T.bblk.data(); G.nblk = nblk;
+    G.bsup = T.bsup.data(); G.nsup = nsup;
+}
+
+// k_row_start, one wavefront serialised: winding terms summed in lane order
+// (exact in any order), secant, exhaustive fallback = first containing cell.
+int32_t row_start(const Pass& P, int64_t row, int64_t* anomalies)
+{
+    const StaticGrid& s = P.st;
+    double px = sweep_x(P, row, 0), py = sweep_y(P, row, 0);
+    if (!(s.bbox[0] <= px && px <= s.bbox[2] && s.bbox[1] <= py &&
+          py <= s.bbox[3]))
+        return STATE_OUTSIDE;
+    double w = 0;
+    for (int64_t k = 0; k < s.nbp; k++) {
+        int64_t km = k == 0 ? s.nbp - 1 : k - 1;
+        w += winding_term(px, py, s.bpx[km], s.bpy[km], s.bpx[k], s.bpy[k]);
+    }
+    if (w == 0) return STATE_OUTSIDE;
+    int64_t c = index_secant(s.x, s.y, s.nx, s.ny, px, py);
+    if (c == RG_SEARCH_BRUTE) {
+        c = -1;
+        int64_t ncy = s.ny - 1, nc = (s.nx - 1) * ncy;
+        for (int64_t q = 0; q < nc; q++)
+            if (cell_contains(s.x, s.y, s.ny, q / ncy, q % ncy, px, py)) {
+                c = q;
+                break;
+            }
+    }
+    if (c < 0) {
+        (*anomalies)++;
+        return STATE_OUTSIDE;
+    }
+    return (int32_t)c;
+}
+
+void chunk_range(const Build& B, const Pass& P, int64_t c, int64_t* k0,
+                 int64_t* k1)
+{
+    *k0 = c * B.K;
+    int64_t e = *k0 + B.K;
+    *k1 = e < P.nk - 1 ? e : P.nk - 1;
+    if (*k1 < *k0) *k1 = *k0;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Emulated build.  Outputs (malloc'd, free with rge_free):
+//   flat_*   uncoalesced triples in emission-ordinal order (n_flat)
+//   csr_*    coalesced triples sorted by (in, out) (n_csr)
+// stats: [n_chunks, n_fixups, n_anomalies, n_long_buckets, failed]
+// `guess` selects the chunk start guesses: 0 = the GPU's cell-search guess,
+// 1 = always "outside" (every chunk of a row that starts inside must then be
+// repaired by the fix-up pass), 2 = pseudo-random garbage states.
+int64_t rge_build(const double* xi, const double* yi, int64_t nxi, int64_t nyi,
+                  const double* xo, const double* yo, int64_t nxo, int64_t nyo,
+                  const double* wi, int64_t K, int guess, int64_t** flat_in,
+                  int64_t** flat_out, double** flat_w, int64_t* n_flat,
+                  int64_t** csr_in, int64_t** csr_out, double** csr_w,
+                  int64_t* stats)
+{
+    const int64_t n_in = (nxi - 1) * (nyi - 1), n_out = (nxo - 1) * (nyo - 1);
+    std::vector<double> vol(n_in);
+    for (int64_t c = 0; c < n_in; c++)
+        vol[c] = cell_area(xi, yi, nyi, c / (nyi - 1), c % (nyi - 1));
+    Tables Ti, To;
+    StaticGrid Gi, Go;
+    prepare(xi, yi, nxi, nyi, Ti, Gi);
+    prepare(xo, yo, nxo, nyo, To, Go);
+
+    Build B;
+    B.vol_in = vol.data();
+    B.wi = wi;
+    B.K = K;
+    int64_t chunk = 0, rows = 0;
+    for (int p = 0; p < 4; p++) {
+        Pass& P = B.pass[p];
+        const int sweep_input = p >= 2, axis = p & 1;
+        const int64_t nxs = sweep_input ? nxi : nxo, nys = sweep_input ? nyi : nyo;
+        P.xs = sweep_input ? xi : xo;
+        P.ys = sweep_input ? yi : yo;
+        if (axis == 0) { P.sr = 1; P.sk = nys; P.nrows = nys; P.nk = nxs; }
+        else { P.sr = nys; P.sk = 1; P.nrows = nxs; P.nk = nys; }
+        P.ncy_sweep = nys - 1;
+        P.axis = axis;
+        P.sweep_input = sweep_input;
+        P.st = sweep_input ? Go : Gi;
+        P.cpr = std::max<int64_t>(1, (P.nk - 1 + K - 1) / K);
+        P.chunk_base = chunk;
+        B.row_base[p] = rows;
+        chunk += P.nrows * P.cpr;
+        rows += P.nrows;
+    }
+    B.nchunks = chunk;
+    B.nrows_total = rows;
+
+    std::vector<int32_t> start(chunk), end(chunk), count(chunk), hist(n_in, 0);
+    int64_t anomalies = 0, fixups = 0, failed = 0, nlong = 0;
+    uint64_t lcg = 0x9E3779B97F4A7C15ull;
+    for (int64_t r = 0; r < rows; r++) {
+        int p = pass_of_row(B, r);
+        const Pass& P = B.pass[p];
+        int64_t row = r - B.row_base[p];
+        start[P.chunk_base + row * P.cpr] = row_start(P, row, &anomalies);
+    }
+    for (int64_t g = 0; g < chunk; g++) {
+        int p = pass_of_chunk(B, g);
+        const Pass& P = B.pass[p];
+        int64_t local = g - P.chunk_base, row = local / P.cpr, c = local % P.cpr;
+        if (c == 0) continue;
+        if (guess == 0) {
+            start[g] = chunk_start_guess(P, row, c * K);
+        } else if (guess == 1) {
+            start[g] = STATE_OUTSIDE;
+        } else {
+            lcg = lcg * 6364136223846793005ull + 1442695040888963407ull;
+            int64_t ncells = (P.st.nx - 1) * (P.st.ny - 1);
+            start[g] = (int32_t)((int64_t)(lcg >> 33) % (ncells + 1)) - 1;
+        }
+    }
+    // count
+    for (int64_t g = 0; g < chunk; g++) {
+        int p = pass_of_chunk(B, g);
+        const Pass& P = B.pass[p];
+        int64_t local = g - P.chunk_base, row = local / P.cpr, c = local % P.cpr;
+        int64_t k0, k1;
+        chunk_range(B, P, c, &k0, &k1);
+        HCount s{0, hist.data()};
+        end[g] = walk_chunk(B, P, row, k0, k1, start[g], s);
+        count[g] = s.n;
+    }
+    // fix-up (k_fixup)
+    for (int64_t r = 0; r < rows; r++) {
+        int p = pass_of_row(B, r);
+        const Pass& P = B.pass[p];
+        int64_t row = r - B.row_base[p], g0 = P.chunk_base + row * P.cpr;
+        for (int64_t c = 1; c < P.cpr; c++) {
+            int64_t g = g0 + c;
+            int32_t truth = end[g - 1];
+            if (truth == STATE_FAILED) break;
+            if (start[g] == truth) continue;
+            int64_t k0, k1;
+            chunk_range(B, P, c, &k0, &k1);
+            HUncount u{hist.data()};
+            (void)walk_chunk(B, P, row, k0, k1, start[g], u);
+            HCount rc{0, hist.data()};
+            end[g] = walk_chunk(B, P, row, k0, k1, truth, rc);
+            count[g] = rc.n;
+            start[g] = truth;
+            fixups++;
+        }
+        for (int64_t c = 0; c < P.cpr; c++)
+            if (end[g0 + c] == STATE_FAILED) {
+                failed++;
+                break;
+            }
+    }
+    std::vector<int64_t> choff(chunk + 1, 0), boff(n_in + 1, 0);
+    for (int64_t g = 0; g < chunk; g++) choff[g + 1] = choff[g] + count[g];
+    for (int64_t i = 0; i < n_in; i++) boff[i + 1] = boff[i] + hist[i];
+    const int64_t T = choff[chunk];
+    // flat emission in ordinal order (for comparison with the oracle sweep)
+    int64_t* fi = (int64_t*)malloc(sizeof(int64_t) * (T ? T : 1));
+    int64_t* fo = (int64_t*)malloc(sizeof(int64_t) * (T ? T : 1));
+    double* fw = (double*)malloc(sizeof(double) * (T ? T : 1));
+    std::vector<uint64_t> key(T ? T : 1);
+    std::vector<double> val(T ? T : 1);
+    std::vector<int32_t> cursor(n_in, 0);
+    for (int64_t g = 0; g < chunk; g++) {
+        int p = pass_of_chunk(B, g);
+        const Pass& P = B.pass[p];
+        int64_t local = g - P.chunk_base, row = local / P.cpr, c = local % P.cpr;
+        int64_t k0, k1;
+        chunk_range(B, P, c, &k0, &k1);
+        HFlat f{choff[g], fi, fo, fw};
+        (void)walk_chunk(B, P, row, k0, k1, start[g], f);
+        HBucket hb{(uint32_t)choff[g], boff.data(), cursor.data(), key.data(),
+                   val.data()};
+        (void)walk_chunk(B, P, row, k0, k1, start[g], hb);
+    }
+    // reduce
+    std::vector<int64_t> nnz_row(n_in);
+    for (int64_t i = 0; i < n_in; i++) {
+        bool lr = false;
+        int64_t s = boff[i], n = boff[i + 1] - s;
+        nnz_row[i] = bucket_reduce(key.data() + s, val.data() + s, n, &lr);
+        if (lr) {
+            nlong++;
+            (void)bucket_reduce_long(key.data() + s, val.data() + s, n);
+        }
+    }
+    int64_t nnz = 0;
+    for (int64_t i = 0; i < n_in; i++) nnz += nnz_row[i];
+    int64_t* ci = (int64_t*)malloc(sizeof(int64_t) * (nnz ? nnz : 1));
+    int64_t* co = (int64_t*)malloc(sizeof(int64_t) * (nnz ? nnz : 1));
+    double* cw = (double*)malloc(sizeof(double) * (nnz ? nnz : 1));
+    int64_t q = 0;
+    for (int64_t i = 0; i < n_in; i++)
+        for (int64_t a = 0; a < nnz_row[i]; a++) {
+            ci[q] = i;
+            co[q] = (int64_t)key[boff[i] + a];
+            cw[q] = val[boff[i] + a];
+            q++;
+        }
+    *flat_in = fi;
+    *flat_out = fo;
+    *flat_w = fw;
+    *n_flat = T;
+    *csr_in = ci;
+    *csr_out = co;
+    *csr_w = cw;
+    stats[0] = chunk;
+    stats[1] = fixups;
+    stats[2] = anomalies;
+    stats[3] = nlong;
+    stats[4] = failed;
+    (void)n_out;
+    return nnz;
+}
+
+void rge_free(void* p) { free(p); }
+
+double rge_reduceat_segment(const double* a, int64_t n)
+{
+    return reduceat_segment(a, n);
+}
+
+void rge_cell_area(const double* x, const double* y, int64_t nx, int64_t ny,
+                   double* out)
+{
+    for (int64_t c = 0; c < (nx - 1) * (ny - 1); c++)
+        out[c] = cell_area(x, y, ny, c / (ny - 1), c % (ny - 1));
+}
+
+int64_t rge_index_secant(const double* x, const double* y, int64_t nx,
+                         int64_t ny, double px, double py)
+{
+    return index_secant(x, y, nx, ny, px, py);
+}
+
+}  // extern "C"

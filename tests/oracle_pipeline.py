@@ -1,0 +1,65 @@
+"""
+The reference's public pipeline with the oracle as the compute backend (test
+infrastructure).
+
+``weights`` / ``regrid_from_weights`` here run the product's host-side
+normalisation (``regridding_amd._util``, ``regridding_amd._regrid``'s
+prepare/finish) around the oracle's sweep, coalesce and apply, so the
+reference's own known-answer tests can pin the oracle, and the GPU drop-in
+can be compared against the same pipeline.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from oracle import oracle
+from regridding_amd import _util
+from regridding_amd._regrid import finish_apply, prepare_apply
+
+
+def weights(coordinates_input, coordinates_output, axis_input=None, axis_output=None,
+            weights_input=None, perturb=None, seed=_util.SEED_DEFAULT):
+    """Conservative 2-D weights, normalised as ``_weights_conservative``."""
+    perturb = _util.resolve_perturb(perturb, coordinates_input)
+    (ci, co, axis_input, axis_output, shape_in, shape_out, shape_orth) = (
+        _util.normalize_coordinates(coordinates_input, coordinates_output, axis_input,
+                                    axis_output, perturb, seed)
+    )
+    sv_in = list(shape_in)
+    for ax in axis_input:
+        sv_in[ax] -= 1
+    sv_out = list(shape_out)
+    for ax in axis_output:
+        sv_out[ax] -= 1
+    if weights_input is not None:
+        weights_input = np.broadcast_to(weights_input, tuple(sv_in))
+    assert len(axis_input) == 2
+    result = np.empty(shape_orth, dtype=object)
+    for index in np.ndindex(*shape_orth):
+        sl_in = _util.orthogonal_slice(index, axis_input)
+        sl_out = _util.orthogonal_slice(index, axis_output)
+        wi = weights_input[index] if weights_input is not None else None
+        result[index] = oracle.weights_2d((ci[0][sl_in], ci[1][sl_in]),
+                                          (co[0][sl_out], co[1][sl_out]), wi)
+    return result, tuple(sv_in), tuple(sv_out)
+
+
+def regrid_from_weights(weights, shape_input, shape_output, values_input, values_output=None,
+                        axis_input=None, axis_output=None):
+    prep = prepare_apply(weights, shape_input, shape_output, values_input, values_output,
+                         axis_input, axis_output)
+    x = prep["values_input"]
+    out = np.zeros((x.shape[0], prep["n_out"]))
+    for d in range(x.shape[0]):
+        ii, io, w = prep["flat"][d]
+        out[d] = oracle.apply(ii, io, np.asarray(w), x[d:d + 1], prep["n_out"])[0]
+    return finish_apply(prep, out)
+
+
+def regrid(coordinates_input, coordinates_output, values_input, values_output=None,
+           axis_input=None, axis_output=None, perturb=None, seed=_util.SEED_DEFAULT):
+    w = weights(coordinates_input, coordinates_output, axis_input, axis_output,
+                perturb=perturb, seed=seed)
+    return regrid_from_weights(*w, values_input=values_input, values_output=values_output,
+                               axis_input=axis_input, axis_output=axis_output)

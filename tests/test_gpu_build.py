@@ -1,0 +1,199 @@
+"""
+GPU parity of the conservative 2-D build, the apply and the searches against
+the CPU oracle (oracle/rg_oracle.c), through the C ABI.
+
+Bar: indices, sparsity and FP64 weights/outputs bit-identical to the oracle
+(exact-order IEEE arithmetic on both sides); FP32 apply within a stated
+relative tolerance of the FP64 result.
+"""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from regridding_amd import _util, workloads
+from regridding_amd._conservative import build_2d
+from regridding_amd._lib import DeviceArray, check, lib
+from regridding_amd._regrid import import_weights
+
+pytestmark = pytest.mark.gpu
+
+
+def _perturbed(grid_in, grid_out, perturb=True, seed=42):
+    ci, co, *_ = _util.normalize_coordinates(grid_in, grid_out, perturb=perturb, seed=seed)
+    return ci, co
+
+
+CASES = {
+    "c1": lambda: workloads.plain_small()[:2],
+    "coarse_to_fine": lambda: (workloads.warped_grid(17, 0.2, 0.03, 0.04),
+                               workloads.warped_grid(61, 0.45, 0.0, 0.0, -0.9, 0.9)),
+    "fine_to_coarse": lambda: (workloads.warped_grid(90, 0.7, 0.06, 0.02),
+                               workloads.warped_grid(12, 0.05, 0.0, 0.0, -0.8, 0.8)),
+    "disjoint_partial": lambda: (workloads.warped_grid((40, 31), 0.3, 0.05, 0.05),
+                                 workloads.warped_grid((27, 35), -0.2, 0.01, 0.0, 0.2, 1.6)),
+    "output_covers_input": lambda: (workloads.warped_grid(20, 0.1, 0.02, 0.02, -0.5, 0.5),
+                                    workloads.warped_grid(30, 0.3, 0.0, 0.0, -1.5, 1.5)),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_build_matches_oracle(case):
+    print(f"case {case}", flush=True)
+    ci, co = _perturbed(*CASES[case]())
+    h = build_2d(ci, co)
+    gi, go, gw = h.export()
+    oi, oo, ow = oracle.weights_2d(ci, co)
+    assert np.array_equal(gi, oi)
+    assert np.array_equal(go, oo)
+    assert np.array_equal(gw, ow)
+    st = h.stats()
+    ti, to, tw = oracle.sweep_2d(ci, co)
+    assert st["n_triples"] == ti.size
+
+
+@pytest.mark.parametrize("K", [1, 3, 16, 64, 4096])
+@pytest.mark.parametrize("block", [64, 256])
+def test_build_tuning_invariant(K, block):
+    ci, co = _perturbed(*CASES["c1"]())
+    ref = build_2d(ci, co).export()
+    got = build_2d(ci, co, edges_per_lane=K, block=block).export()
+    for a, b in zip(ref, got):
+        assert np.array_equal(a, b)
+
+
+def test_build_weights_input():
+    ci, co = _perturbed(*CASES["coarse_to_fine"]())
+    wi = np.random.default_rng(7).uniform(0.5, 3.0, (ci[0].shape[0] - 1, ci[0].shape[1] - 1))
+    got = build_2d(ci, co, wi).export()
+    ref = oracle.weights_2d(ci, co, wi)
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b)
+
+
+def test_build_unperturbed_rectilinear():
+    # perturb=False, exactly rectilinear grids with shared edge coordinates:
+    # the degenerate geometry the reference tests use
+    x = np.linspace(-1, 1, 8)
+    y = np.linspace(-1, 1, 12)
+    gi = np.meshgrid(x, y, indexing="ij")
+    go = (gi[0] + 0.03, gi[1] + 0.02)
+    ci, co = _perturbed(gi, go, perturb=False)
+    got = build_2d(ci, co).export()
+    ref = oracle.weights_2d(ci, co)
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b)
+
+
+def test_build_deterministic():
+    ci, co = _perturbed(*CASES["fine_to_coarse"]())
+    a = build_2d(ci, co).export()
+    b = build_2d(ci, co).export()
+    for p, q in zip(a, b):
+        assert np.array_equal(p, q)
+
+
+def test_cell_area_matches_oracle():
+    x, y = workloads.warped_grid((57, 43), 0.4, 0.07, 0.03)
+    dx, dy = DeviceArray.from_host(x), DeviceArray.from_host(y)
+    out = DeviceArray.empty((56 * 42,), np.float64)
+    check(lib().rg_cell_area_2d(dx.ptr, dy.ptr, 57, 43, out.ptr, 0))
+    assert np.array_equal(out.to_host().reshape(56, 42), oracle.grid_volume(x, y))
+
+
+def _apply(h, x, transpose=False, f32=False):
+    D = x.shape[0]
+    n_out = h.n_in if transpose else h.n_out
+    dt = np.float32 if f32 else np.float64
+    dx = DeviceArray.from_host(np.ascontiguousarray(x, dtype=dt))
+    dy = DeviceArray.empty((D, n_out), dt)
+    fn = lib().rg_apply_f32 if f32 else lib().rg_apply_f64
+    check(fn(h.ptr, int(transpose), dx.ptr, dy.ptr, D, 0))
+    return dy.to_host()
+
+
+@pytest.mark.parametrize("D", [1, 3, 4, 16, 21])
+def test_apply_matches_oracle(D):
+    ci, co = _perturbed(*CASES["c1"]())
+    h = build_2d(ci, co)
+    ii, io, w = h.export()
+    x = np.random.default_rng(D).poisson(50, (D, h.n_in)).astype(np.float64)
+    assert np.array_equal(_apply(h, x), oracle.apply(ii, io, w, x, h.n_out))
+    # adjoint: the reference applied to the swapped triples (transpose_weights)
+    y = np.random.default_rng(D + 1).normal(size=(D, h.n_out))
+    assert np.array_equal(_apply(h, y, transpose=True), oracle.apply(io, ii, w, y, h.n_in))
+
+
+def test_apply_f32_tolerance():
+    ci, co = _perturbed(*CASES["c1"]())
+    h = build_2d(ci, co)
+    ii, io, w = h.export()
+    x = np.random.default_rng(3).poisson(50, (4, h.n_in)).astype(np.float64)
+    ref = oracle.apply(ii, io, w, x, h.n_out)
+    got = _apply(h, x, f32=True)
+    # FP32 values, weights and accumulation over ~5 terms: a few ulp of fp32
+    assert np.max(np.abs(got - ref)) <= 1e-5 * np.max(np.abs(ref))
+
+
+def test_import_any_order_matches_reference_scatter():
+    ci, co = _perturbed(*CASES["coarse_to_fine"]())
+    ii, io, w = oracle.weights_2d(ci, co)
+    n_in = (ci[0].shape[0] - 1) * (ci[0].shape[1] - 1)
+    n_out = (co[0].shape[0] - 1) * (co[0].shape[1] - 1)
+    perm = np.random.default_rng(5).permutation(ii.size)
+    # storage order matters: duplicates and a shuffled order are summed in order
+    ii2 = np.concatenate([ii[perm], ii[:50]])
+    io2 = np.concatenate([io[perm], io[:50]])
+    w2 = np.concatenate([w[perm], -0.5 * w[:50]])
+    h = import_weights(ii2, io2, w2, n_in, n_out)
+    x = np.random.default_rng(9).normal(size=(3, n_in))
+    assert np.array_equal(_apply(h, x), oracle.apply(ii2, io2, w2, x, n_out))
+    y = np.random.default_rng(10).normal(size=(2, n_out))
+    assert np.array_equal(_apply(h, y, transpose=True), oracle.apply(io2, ii2, w2, y, n_in))
+
+
+def test_import_rejects_out_of_range():
+    with pytest.raises(RuntimeError, match="out of range"):
+        import_weights(np.array([0, 5]), np.array([0, 1]), np.array([1.0, 1.0]), 3, 3)
+
+
+def test_find_cells_2d_matches_oracle_secant():
+    x, y = workloads.warped_grid((30, 24), 0.3, 0.05, 0.05)
+    rng = np.random.default_rng(11)
+    px = rng.uniform(-1.6, 1.6, 500)
+    py = rng.uniform(-1.6, 1.6, 500)
+    # include vertices and edge midpoints (points on cell faces)
+    px = np.concatenate([px, x[3:6, 4:9].ravel(), 0.5 * (x[5, 2:7] + x[6, 2:7])])
+    py = np.concatenate([py, y[3:6, 4:9].ravel(), 0.5 * (y[5, 2:7] + y[6, 2:7])])
+    n = px.size
+    dx, dy, dpx, dpy = (DeviceArray.from_host(a) for a in (x, y, px, py))
+    dij = DeviceArray.empty((n, 2), np.int64)
+    check(lib().rg_find_cells_2d(dx.ptr, dy.ptr, 30, 24, dpx.ptr, dpy.ptr, n, dij.ptr, 0))
+    got = dij.to_host()
+    ref = np.array([oracle.index_of_point_secant((a, b), (x, y)) for a, b in zip(px, py)])
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_find_indices_1d_matches_oracle(method):
+    rng = np.random.default_rng(12)
+    xin = np.sort(rng.uniform(-1, 1, (3, 40)), axis=1)
+    xout = rng.uniform(-1.3, 1.3, (3, 77))
+    xout[:, :3] = xin[:, :3]  # exact hits, including x[0]
+    da, dv = DeviceArray.from_host(xin), DeviceArray.from_host(xout)
+    dr = DeviceArray.empty((3, 77), np.int64)
+    check(lib().rg_find_indices_1d(da.ptr, 40, dv.ptr, 77, 3, method, -7, dr.ptr, 0))
+    ref = oracle.find_indices_1d(xin, xout, -7, ["brute", "searchsorted"][method])
+    assert np.array_equal(dr.to_host(), ref)
+
+
+def test_injected_failure_is_reported_and_recovers():
+    ci, co = _perturbed(*CASES["c1"]())
+    lib().rg_inject_failure(b"build")
+    with pytest.raises(RuntimeError, match="injected"):
+        build_2d(ci, co)
+    # the next build is unaffected
+    assert build_2d(ci, co).nnz > 0
+    lib().rg_inject_failure(None)
