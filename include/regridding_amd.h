@@ -42,7 +42,9 @@ typedef struct rg_tuning {
     int32_t edges_per_lane; /* sweep edges walked by one lane (default 16) */
     int32_t block;          /* threads per block of the walk kernels (128) */
     int32_t flags;          /* RG_TUNE_TIMINGS: record per-phase times */
-    int32_t reserved;
+    int32_t stage_per_edge; /* triples staged per sweep edge by the counting
+                               walk (default 8); chunks that emit more are
+                               walked a second time */
 } rg_tuning;
 
 #define RG_TUNE_TIMINGS 1
@@ -55,6 +57,7 @@ typedef struct rg_build_stats {
     int64_t n_subsegments;  /* inside sub-segments (the FP64 work unit) */
     int64_t n_search_anomalies; /* row starts inside the boundary polygon
                                    that the cell search did not place */
+    int64_t n_restaged;     /* chunks whose triples overflowed the staging */
     double ms_prepare, ms_start, ms_count, ms_fixup, ms_emit, ms_reduce,
         ms_transpose, ms_total;
 } rg_build_stats;
@@ -77,6 +80,16 @@ int rg_stream_create(void** stream);
 int rg_stream_destroy(void* stream);
 int rg_stream_sync(void* stream);
 int rg_device_sync(void);
+
+/* HIP event timers (for hosts that time kernels on their own stream). */
+int rg_event_create(void** event);
+int rg_event_destroy(void* event);
+int rg_event_record(void* event, void* stream);
+int rg_event_elapsed_ms(void* start, void* stop, float* ms); /* waits on stop */
+
+/* Plain device copy of n doubles through a streaming kernel (no reference
+ * counterpart): the known-size reference for hardware traffic counters. */
+int rg_copy_f64(const double* src, double* dst, int64_t n, void* stream);
 
 /* Replaces _weights_conservative_2d/_grids.py:50 grid_volume: signed
  * shoelace area of every cell of an (nx, ny) vertex grid, accumulated in the
@@ -117,6 +130,18 @@ int rg_weights_import(const int64_t* indices_input,
                       rg_weights_t* out, void* stream);
 
 int rg_weights_free(rg_weights_t w);
+
+/* Declares the 2-D cell shapes of imported weights (built weights know
+ * them): the apply walks its outputs in 2-D tiles for cache locality.  Has
+ * no effect on results. */
+int rg_weights_set_shape(rg_weights_t w, int64_t ncx_in, int64_t ncy_in,
+                         int64_t ncx_out, int64_t ncy_out);
+
+/* Launch shape of the apply kernels (process-wide; 0 keeps the current
+ * value): tile_w output cells along the last axis per tile row of a
+ * 256-cell tile, and images_per_pass images accumulated per pass over the
+ * weights (1, 2, 4, 8 or 16).  Has no effect on results. */
+int rg_set_apply_tuning(int tile_w, int images_per_pass);
 
 /* Replaces _regrid/_regrid_from_weights.py:166 _regrid_from_weights for
  * `batch` elements that share one weight matrix: values_input is

@@ -90,6 +90,12 @@ def find_indices(
     if fill_value is None:
         fill_value = np.iinfo(int).max
 
+    if method == "secant" and len(coordinates_input) == 2:
+        # the 2-D extension keeps the grid's own axis order (ascending), so
+        # (i, j) index the input arrays as given
+        axis_input_ = tuple(sorted(axis_input_))
+        axis_output_ = tuple(sorted(axis_output_))
+
     axis_input_last = ~np.arange(len(axis_input_))[::-1]
     axis_output_last = ~np.arange(len(axis_output_))[::-1]
     shape_input_last = tuple(shape_input[ax] for ax in axis_input_)

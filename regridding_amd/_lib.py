@@ -21,6 +21,7 @@ __all__ = [
     "check",
     "DeviceArray",
     "Weights",
+    "EventTimer",
     "LIB_PATH",
 ]
 
@@ -44,7 +45,7 @@ class rg_tuning(ctypes.Structure):
         ("edges_per_lane", ctypes.c_int32),
         ("block", ctypes.c_int32),
         ("flags", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("stage_per_edge", ctypes.c_int32),
     ]
 
 
@@ -55,6 +56,7 @@ class rg_build_stats(ctypes.Structure):
         ("n_fixups", ctypes.c_int64),
         ("n_subsegments", ctypes.c_int64),
         ("n_search_anomalies", ctypes.c_int64),
+        ("n_restaged", ctypes.c_int64),
         ("ms_prepare", ctypes.c_double),
         ("ms_start", ctypes.c_double),
         ("ms_count", ctypes.c_double),
@@ -87,6 +89,11 @@ SIGNATURES = {
     "rg_stream_destroy": (ctypes.c_int, [_vp]),
     "rg_stream_sync": (ctypes.c_int, [_vp]),
     "rg_device_sync": (ctypes.c_int, []),
+    "rg_event_create": (ctypes.c_int, [ctypes.POINTER(_vp)]),
+    "rg_event_destroy": (ctypes.c_int, [_vp]),
+    "rg_event_record": (ctypes.c_int, [_vp, _vp]),
+    "rg_event_elapsed_ms": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_float)]),
+    "rg_copy_f64": (ctypes.c_int, [_vp, _vp, _i64, _vp]),
     "rg_cell_area_2d": (ctypes.c_int, [_vp, _vp, _i64, _i64, _vp, _vp]),
     "rg_build_conservative_2d": (
         ctypes.c_int,
@@ -97,6 +104,8 @@ SIGNATURES = {
     "rg_weights_export": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "rg_weights_import": (ctypes.c_int, [_vp, _vp, _vp, _i64, _i64, _i64, ctypes.POINTER(_vp), _vp]),
     "rg_weights_free": (ctypes.c_int, [_vp]),
+    "rg_weights_set_shape": (ctypes.c_int, [_vp, _i64, _i64, _i64, _i64]),
+    "rg_set_apply_tuning": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "rg_apply_f64": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _i64, _vp]),
     "rg_apply_f32": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _i64, _vp]),
     "rg_find_cells_2d": (ctypes.c_int, [_vp, _vp, _i64, _i64, _vp, _vp, _i64, _vp, _vp]),
@@ -183,6 +192,32 @@ class DeviceArray:
 
     def free(self) -> None:
         self._fin()
+
+
+class EventTimer:
+    """A pair of HIP events recorded on one stream."""
+
+    def __init__(self, stream: int = 0):
+        self.stream = stream
+        a, b = _vp(), _vp()
+        check(lib().rg_event_create(ctypes.byref(a)))
+        check(lib().rg_event_create(ctypes.byref(b)))
+        self.a, self.b = a.value, b.value
+
+    def start(self) -> None:
+        check(lib().rg_event_record(self.a, self.stream))
+
+    def stop(self) -> float:
+        """Record the stop event, wait for it, return milliseconds."""
+        check(lib().rg_event_record(self.b, self.stream))
+        ms = ctypes.c_float()
+        check(lib().rg_event_elapsed_ms(self.a, self.b, ctypes.byref(ms)))
+        return float(ms.value)
+
+    def __del__(self):
+        if _lib is not None:
+            _lib.rg_event_destroy(self.a)
+            _lib.rg_event_destroy(self.b)
 
 
 def _finalize_weights(ptr: int) -> None:

@@ -15,10 +15,16 @@
 // rows hold the outputs in ascending order: exactly the order in which the
 // reference applies regrid_from_weights(*transpose_weights(w)).
 //
-// Launch layout: 256 rows per block; blocks are renumbered so that each of
-// the 8 XCDs of the MI355X (which receive blocks round-robin) works on one
-// contiguous range of rows, keeping the gathered input neighbourhood in that
-// XCD's L2.
+// Launch layout.  The rows of a CSR are the cells of a 2-D grid.  A block
+// of 256 lanes owns a tile of output cells, tile_w wide along the grid's
+// last axis and 256 / tile_w tall, so the input cells it gathers form one
+// compact patch (a linear run of rows would be a thin band tilted across
+// the input grid, re-fetched by every neighbouring run).  Blocks are
+// renumbered so each of the 8 XCDs of the MI355X (blocks arrive
+// round-robin) sweeps one contiguous band of tiles, keeping the patches of
+// neighbouring tiles in that XCD's L2.  The launch's second dimension is
+// the image group: all tiles of images [0, DB) first, then the next DB, so
+// the gathered footprint per pass is DB images deep.
 #include <hip/hip_runtime.h>
 
 #include <mutex>
@@ -30,37 +36,203 @@ namespace rg {
 constexpr int APPLY_BLOCK = 256;
 constexpr int NUM_XCD = 8;
 
+struct ApplyGrid {
+    int64_t gx, gy;       // rows of the CSR as a gx x gy grid of cells
+    int64_t tw, th;       // tile shape (tw * th = APPLY_BLOCK)
+    int64_t ntx, ntiles;  // tiles along gy, tiles in total
+};
+
 template <class T, int DB>
 __global__ __launch_bounds__(APPLY_BLOCK) void k_apply(
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
     const T* __restrict__ w, const T* __restrict__ x, T* __restrict__ y,
-    int64_t nrows, int64_t ncols, int64_t batch, int64_t ntiles)
+    int64_t nrows, int64_t ncols, int64_t batch, ApplyGrid G)
 {
     const int64_t nb = gridDim.x;  // a multiple of NUM_XCD
     const int64_t b = blockIdx.x;
     const int64_t tile = (b % NUM_XCD) * (nb / NUM_XCD) + b / NUM_XCD;
-    if (tile >= ntiles) return;
-    const int64_t r = tile * APPLY_BLOCK + threadIdx.x;
-    if (r >= nrows) return;
+    if (tile >= G.ntiles) return;
+    const int64_t ti = tile / G.ntx, tj = tile % G.ntx;
+    const int64_t i = ti * G.th + threadIdx.x / G.tw;
+    const int64_t j = tj * G.tw + threadIdx.x % G.tw;
+    if (i >= G.gx || j >= G.gy) return;
+    const int64_t r = i * G.gy + j;
+    const int64_t d0 = (int64_t)blockIdx.y * DB;
+    const int64_t nd = batch - d0 < DB ? batch - d0 : DB;
     const int32_t s = rowptr[r], e = rowptr[r + 1];
-    for (int64_t d0 = 0; d0 < batch; d0 += DB) {
-        const int64_t nd = batch - d0 < DB ? batch - d0 : DB;
-        T acc[DB];
+    T acc[DB];
 #pragma unroll
-        for (int j = 0; j < DB; j++) acc[j] = T(0);
+    for (int q = 0; q < DB; q++) acc[q] = T(0);
+    const T* xb = x + d0 * ncols;
+    for (int32_t q = s; q < e; q++) {
+        const int64_t c = col[q];
+        const T wq = w[q];
+#pragma unroll
+        for (int m = 0; m < DB; m++)
+            if (m < nd) acc[m] = acc[m] + wq * xb[m * ncols + c];
+    }
+    T* yb = y + d0 * nrows + r;
+#pragma unroll
+    for (int m = 0; m < DB; m++)
+        if (m < nd) yb[m * nrows] = acc[m];
+}
+
+// ---- the apply plan -------------------------------------------------------
+//
+// With many images per pass, one tile's gathered footprint (patch cells x
+// images x 8 B) outgrows the L1, and the tiles in flight on an XCD outgrow
+// its L2: a cache line of the input is evicted before the neighbouring cells
+// it also holds are used, and comes back from HBM several times.  The plan
+// removes the dependence on cache luck.  Made once per handle and direction
+// (the weights are static; every later apply reuses it), it stores for every
+// tile the bounding box (i0, j0, rows, cols) of the input cells its rows
+// gather, and for every CSR entry the entry's offset inside that box.  The
+// planned apply then stages the box of DB images in LDS with one coalesced
+// row load per box row, and gathers from LDS.
+
+constexpr int LDS_BYTES = 64 * 1024;  // staged patch per block
+
+__global__ __launch_bounds__(APPLY_BLOCK) void k_plan(
+    const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+    ApplyGrid G, int64_t src_gy, int32_t* __restrict__ box,
+    uint16_t* __restrict__ off)
+{
+    __shared__ int32_t lo_i, hi_i, lo_j, hi_j;
+    if (threadIdx.x == 0) {
+        lo_i = INT32_MAX; hi_i = INT32_MIN; lo_j = INT32_MAX; hi_j = INT32_MIN;
+    }
+    __syncthreads();
+    const int64_t tile = blockIdx.x;
+    const int64_t ti = tile / G.ntx, tj = tile % G.ntx;
+    const int64_t i = ti * G.th + threadIdx.x / G.tw;
+    const int64_t j = tj * G.tw + threadIdx.x % G.tw;
+    const bool valid = i < G.gx && j < G.gy;
+    const int64_t r = i * G.gy + j;
+    int32_t s = 0, e = 0;
+    if (valid) {
+        s = rowptr[r];
+        e = rowptr[r + 1];
+    }
+    int32_t a0 = INT32_MAX, a1 = INT32_MIN, b0 = INT32_MAX, b1 = INT32_MIN;
+    for (int32_t q = s; q < e; q++) {
+        const int32_t c = col[q];
+        const int32_t ci = (int32_t)(c / src_gy), cj = (int32_t)(c - ci * src_gy);
+        a0 = min(a0, ci); a1 = max(a1, ci);
+        b0 = min(b0, cj); b1 = max(b1, cj);
+    }
+    if (s < e) {
+        atomicMin(&lo_i, a0); atomicMax(&hi_i, a1);
+        atomicMin(&lo_j, b0); atomicMax(&hi_j, b1);
+    }
+    __syncthreads();
+    const bool empty = lo_i > hi_i;
+    const int32_t rows = empty ? 0 : hi_i - lo_i + 1;
+    const int32_t cols = empty ? 0 : hi_j - lo_j + 1;
+    if (threadIdx.x == 0) {
+        box[4 * tile + 0] = empty ? 0 : lo_i;
+        box[4 * tile + 1] = empty ? 0 : lo_j;
+        box[4 * tile + 2] = rows;
+        box[4 * tile + 3] = cols;
+    }
+    // offsets are only read for patches that fit the LDS (< 2^16 cells)
+    const int64_t cells = (int64_t)rows * cols;
+    for (int32_t q = s; q < e; q++) {
+        const int32_t c = col[q];
+        const int32_t ci = (int32_t)(c / src_gy), cj = (int32_t)(c - ci * src_gy);
+        const int64_t o = (int64_t)(ci - lo_i) * cols + (cj - lo_j);
+        off[q] = cells < 65536 ? (uint16_t)o : (uint16_t)0;
+    }
+}
+
+__global__ void k_plan_max(const int32_t* __restrict__ box, int64_t ntiles,
+                           int64_t* __restrict__ out)
+{
+    int64_t m = 0;
+    for (int64_t t = threadIdx.x; t < ntiles; t += blockDim.x) {
+        const int64_t c = (int64_t)box[4 * t + 2] * box[4 * t + 3];
+        m = c > m ? c : m;
+    }
+    __shared__ int64_t sh[256];
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o && sh[threadIdx.x + o] > sh[threadIdx.x])
+            sh[threadIdx.x] = sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
+}
+
+// lds_bytes: the dynamic LDS of the launch; a tile whose patch for the
+// block's images does not fit gathers from global memory instead.
+template <class T, int DB>
+__global__ __launch_bounds__(APPLY_BLOCK) void k_apply_planned(
+    const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+    const uint16_t* __restrict__ off, const int32_t* __restrict__ box,
+    const T* __restrict__ w, const T* __restrict__ x, T* __restrict__ y,
+    int64_t nrows, int64_t ncols, int64_t src_gy, int64_t batch, ApplyGrid G,
+    int64_t lds_bytes)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    T* patch = reinterpret_cast<T*>(lds_raw);
+    const int64_t nb = gridDim.x;  // a multiple of NUM_XCD
+    const int64_t b = blockIdx.x;
+    const int64_t tile = (b % NUM_XCD) * (nb / NUM_XCD) + b / NUM_XCD;
+    if (tile >= G.ntiles) return;  // uniform per block
+    const int32_t i0 = box[4 * tile + 0], j0 = box[4 * tile + 1];
+    const int32_t rows = box[4 * tile + 2], cols = box[4 * tile + 3];
+    const int64_t cells = (int64_t)rows * cols;
+    const int64_t d0 = (int64_t)blockIdx.y * DB;
+    const int64_t nd = batch - d0 < DB ? batch - d0 : DB;
+    const bool staged = cells > 0 && cells * nd * (int64_t)sizeof(T) <= lds_bytes;
+    if (staged) {
+        // box row (image m, row ii) per wavefront; lanes along the row
+        const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        const int32_t ncell = (int32_t)cells;
+        for (int32_t m = 0; m < (int32_t)nd; m++) {
+            const T* src_m = x + (d0 + m) * ncols + (int64_t)i0 * src_gy + j0;
+            T* dst_m = patch + m * ncell;
+            for (int32_t ii = wid; ii < rows; ii += APPLY_BLOCK / 64) {
+                const T* src = src_m + (int64_t)ii * src_gy;
+                T* dst = dst_m + ii * cols;
+                for (int32_t jj = lane; jj < cols; jj += 64) dst[jj] = src[jj];
+            }
+        }
+        __syncthreads();
+    }
+    const int32_t ti = (int32_t)(tile / G.ntx), tj = (int32_t)(tile - (int64_t)ti * G.ntx);
+    const int32_t tw = (int32_t)G.tw;
+    const int64_t i = (int64_t)ti * G.th + (int32_t)threadIdx.x / tw;
+    const int64_t j = (int64_t)tj * G.tw + (int32_t)threadIdx.x % tw;
+    if (i >= G.gx || j >= G.gy) return;
+    const int64_t r = i * G.gy + j;
+    const int32_t s = rowptr[r], e = rowptr[r + 1];
+    T acc[DB];
+#pragma unroll
+    for (int m = 0; m < DB; m++) acc[m] = T(0);
+    if (staged) {
+        const int32_t ncell = (int32_t)cells;
+        for (int32_t q = s; q < e; q++) {
+            const int32_t o = off[q];
+            const T wq = w[q];
+#pragma unroll
+            for (int m = 0; m < DB; m++)
+                if (m < nd) acc[m] = acc[m] + wq * patch[m * ncell + o];
+        }
+    } else {
         const T* xb = x + d0 * ncols;
         for (int32_t q = s; q < e; q++) {
             const int64_t c = col[q];
             const T wq = w[q];
 #pragma unroll
-            for (int j = 0; j < DB; j++)
-                if (j < nd) acc[j] = acc[j] + wq * xb[j * ncols + c];
+            for (int m = 0; m < DB; m++)
+                if (m < nd) acc[m] = acc[m] + wq * xb[m * ncols + c];
         }
-        T* yb = y + d0 * nrows + r;
-#pragma unroll
-        for (int j = 0; j < DB; j++)
-            if (j < nd) yb[j * nrows] = acc[j];
     }
+    T* yb = y + d0 * nrows + r;
+#pragma unroll
+    for (int m = 0; m < DB; m++)
+        if (m < nd) yb[m * nrows] = acc[m];
 }
 
 __global__ void k_to_f32(const double* __restrict__ a, float* __restrict__ b,
@@ -73,25 +245,102 @@ __global__ void k_to_f32(const double* __restrict__ a, float* __restrict__ b,
 
 namespace {
 
+int g_tile_w = 16;          // tuned on the plain 2049^2 -> 1537^2 pair
+int g_images_per_pass = 8;
+bool g_use_plan = true;
+std::mutex g_plan_mu;
+
+ApplyGrid make_grid(const int64_t* shape, int64_t nrows)
+{
+    ApplyGrid G;
+    G.gx = shape[0];
+    G.gy = shape[1];
+    if (G.gx * G.gy != nrows) {  // shape unknown: one long row
+        G.gx = 1;
+        G.gy = nrows;
+    }
+    G.tw = g_tile_w;
+    if (G.gx == 1 || G.tw > G.gy) G.tw = APPLY_BLOCK;
+    G.th = APPLY_BLOCK / G.tw;
+    G.ntx = (G.gy + G.tw - 1) / G.tw;
+    G.ntiles = G.ntx * ((G.gx + G.th - 1) / G.th);
+    return G;
+}
+
+// The plan of one direction for the current tile shape (made on first use
+// on stream s; later applies on other streams must be ordered after it).
+int ensure_plan(rg_weights* W, int dir, const int32_t* rowptr,
+                const int32_t* col, const ApplyGrid& G, int64_t src_gy,
+                hipStream_t s, rg_weights::Plan** out)
+{
+    *out = nullptr;
+    if (!g_use_plan || G.gx == 1 || W->nnz == 0) return RG_OK;
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    rg_weights::Plan& P = W->plan[dir];
+    if (P.tile_w != G.tw || P.ntiles != G.ntiles) {
+        dev_free(P.box);
+        dev_free(P.off);
+        P.box = nullptr;
+        P.off = nullptr;
+        P.tile_w = 0;
+        RG_TRY(dev_alloc((void**)&P.box, 4 * sizeof(int32_t) * G.ntiles, s));
+        RG_TRY(dev_alloc((void**)&P.off, sizeof(uint16_t) * W->nnz, s));
+        k_plan<<<(unsigned)G.ntiles, APPLY_BLOCK, 0, s>>>(rowptr, col, G, src_gy,
+                                                        P.box, P.off);
+        DevBuf<int64_t> mx;
+        RG_TRY(mx.alloc(1, s));
+        k_plan_max<<<1, 256, 0, s>>>(P.box, G.ntiles, mx.p);
+        RG_HIP(hipGetLastError());
+        RG_TRY(read_scalars(mx.p, &P.max_cells, sizeof(int64_t), s));
+        P.tile_w = G.tw;
+        P.ntiles = G.ntiles;
+    }
+    *out = &P;
+    return RG_OK;
+}
+
 template <class T>
-int launch_apply(const int32_t* rowptr, const int32_t* col, const T* w,
-                 const T* x, T* y, int64_t nrows, int64_t ncols, int64_t batch,
-                 hipStream_t s)
+int launch_apply(rg_weights* W, int dir, const int32_t* rowptr,
+                 const int32_t* col, const T* w, const T* x, T* y,
+                 int64_t nrows, int64_t ncols, const int64_t* shape,
+                 int64_t src_gy, int64_t batch, hipStream_t s)
 {
     if (nrows == 0 || batch == 0) return RG_OK;
-    const int64_t ntiles = (nrows + APPLY_BLOCK - 1) / APPLY_BLOCK;
-    const int64_t nb = (ntiles + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
-    dim3 g((unsigned)nb), bl(APPLY_BLOCK);
-    if (batch >= 16)
-        k_apply<T, 16><<<g, bl, 0, s>>>(rowptr, col, w, x, y, nrows, ncols, batch, ntiles);
-    else if (batch >= 8)
-        k_apply<T, 8><<<g, bl, 0, s>>>(rowptr, col, w, x, y, nrows, ncols, batch, ntiles);
-    else if (batch >= 4)
-        k_apply<T, 4><<<g, bl, 0, s>>>(rowptr, col, w, x, y, nrows, ncols, batch, ntiles);
-    else if (batch >= 2)
-        k_apply<T, 2><<<g, bl, 0, s>>>(rowptr, col, w, x, y, nrows, ncols, batch, ntiles);
-    else
-        k_apply<T, 1><<<g, bl, 0, s>>>(rowptr, col, w, x, y, nrows, ncols, batch, ntiles);
+    const ApplyGrid G = make_grid(shape, nrows);
+    rg_weights::Plan* P = nullptr;
+    RG_TRY(ensure_plan(W, dir, rowptr, col, G, src_gy, s, &P));
+    const int64_t nb = (G.ntiles + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
+    int db = g_images_per_pass;
+    while (db > 1 && db > batch) db >>= 1;
+    // staged patches: keep the LDS per block at or under 32 KiB (5+ blocks
+    // per CU) by staging fewer images per pass
+    while (P && db > 1 && P->max_cells * db * (int64_t)sizeof(T) > LDS_BYTES / 2) db >>= 1;
+    dim3 g((unsigned)nb, (unsigned)((batch + db - 1) / db)), bl(APPLY_BLOCK);
+    if (P) {
+        int64_t lds = P->max_cells * db * (int64_t)sizeof(T);
+        lds = (lds + 255) / 256 * 256;
+        if (lds > LDS_BYTES) lds = LDS_BYTES;
+#define RG_PLANNED(D)                                                          \
+    k_apply_planned<T, D><<<g, bl, (size_t)lds, s>>>(rowptr, col, P->off, P->box, \
+                                                     w, x, y, nrows, ncols, src_gy, \
+                                                     batch, G, lds)
+        switch (db) {
+        case 16: RG_PLANNED(16); break;
+        case 8: RG_PLANNED(8); break;
+        case 4: RG_PLANNED(4); break;
+        case 2: RG_PLANNED(2); break;
+        default: RG_PLANNED(1); break;
+        }
+#undef RG_PLANNED
+    } else {
+        switch (db) {
+        case 16: k_apply<T, 16><<<g, bl, 0, s>>>(rowptr, col, w, x, y, nrows, ncols, batch, G); break;
+        case 8: k_apply<T, 8><<<g, bl, 0, s>>>(rowptr, col, w, x, y, nrows, ncols, batch, G); break;
+        case 4: k_apply<T, 4><<<g, bl, 0, s>>>(rowptr, col, w, x, y, nrows, ncols, batch, G); break;
+        case 2: k_apply<T, 2><<<g, bl, 0, s>>>(rowptr, col, w, x, y, nrows, ncols, batch, G); break;
+        default: k_apply<T, 1><<<g, bl, 0, s>>>(rowptr, col, w, x, y, nrows, ncols, batch, G); break;
+        }
+    }
     RG_HIP(hipGetLastError());
     return RG_OK;
 }
@@ -144,12 +393,13 @@ int rg_apply_f64(rg_weights_t w, int transpose, const double* values_input,
     RG_TRY(check_apply(w, values_input, values_output, batch));
     hipStream_t s = (hipStream_t)stream;
     if (transpose)
-        return launch_apply<double>(w->rowptr_in, w->col_in, w->val_in,
+        return launch_apply<double>(w, 1, w->rowptr_in, w->col_in, w->val_in,
                                     values_input, values_output, w->n_in,
-                                    w->n_out, batch, s);
-    return launch_apply<double>(w->rowptr_out, w->col_out, w->val_out,
+                                    w->n_out, w->shape_in, w->shape_out[1],
+                                    batch, s);
+    return launch_apply<double>(w, 0, w->rowptr_out, w->col_out, w->val_out,
                                 values_input, values_output, w->n_out, w->n_in,
-                                batch, s);
+                                w->shape_out, w->shape_in[1], batch, s);
 }
 
 int rg_apply_f32(rg_weights_t w, int transpose, const float* values_input,
@@ -159,12 +409,37 @@ int rg_apply_f32(rg_weights_t w, int transpose, const float* values_input,
     hipStream_t s = (hipStream_t)stream;
     RG_TRY(ensure_f32(w, s));
     if (transpose)
-        return launch_apply<float>(w->rowptr_in, w->col_in, w->val_in_f32,
+        return launch_apply<float>(w, 1, w->rowptr_in, w->col_in, w->val_in_f32,
                                    values_input, values_output, w->n_in,
-                                   w->n_out, batch, s);
-    return launch_apply<float>(w->rowptr_out, w->col_out, w->val_out_f32,
+                                   w->n_out, w->shape_in, w->shape_out[1],
+                                   batch, s);
+    return launch_apply<float>(w, 0, w->rowptr_out, w->col_out, w->val_out_f32,
                                values_input, values_output, w->n_out, w->n_in,
-                               batch, s);
+                               w->shape_out, w->shape_in[1], batch, s);
+}
+
+int rg_set_apply_tuning(int tile_w, int images_per_pass)
+{
+    // negative tile_w: keep the tile width, turn the staged (planned) apply
+    // off; the direct-gather kernel computes the same bits
+    if (tile_w < 0) {
+        g_use_plan = false;
+        tile_w = -tile_w;
+    } else if (tile_w > 0) {
+        g_use_plan = true;
+    }
+    if (tile_w) {
+        if (tile_w < 1 || tile_w > APPLY_BLOCK || APPLY_BLOCK % tile_w)
+            return fail(RG_ERR_ARGUMENT, "tile_w must divide 256");
+        g_tile_w = tile_w;
+    }
+    if (images_per_pass) {
+        if (images_per_pass != 1 && images_per_pass != 2 && images_per_pass != 4 &&
+            images_per_pass != 8 && images_per_pass != 16)
+            return fail(RG_ERR_ARGUMENT, "images_per_pass must be 1, 2, 4, 8 or 16");
+        g_images_per_pass = images_per_pass;
+    }
+    return RG_OK;
 }
 
 }  // extern "C"

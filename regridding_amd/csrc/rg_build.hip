@@ -29,21 +29,39 @@ namespace rg {
 
 // ---- atomics usable from the shared walk code ----------------------------
 
-struct CountSink {
+// Input-cell buckets are split by sweep pass: bucket (cell, pass) at slot
+// 4 * cell + pass, so a cell's four pieces are contiguous (one bucket for
+// the reduction) while the triples one pass writes for neighbouring cells
+// land close together.
+
+// Counting walk: counts the chunk's triples, adds them to the per-input-cell
+// histogram and keeps the first `cap` of them in the staging buffer.
+struct StageSink {
     int32_t n;
-    int32_t* hist;  // per input cell, or null
-    __device__ void emit(int64_t cin, int64_t, double)
+    int64_t g, cap;
+    Triple* __restrict__ stage;
+    int32_t* __restrict__ hist;
+    int32_t p;
+    __device__ void emit(int64_t cin, int64_t cout, double w)
     {
+        if (n < cap) {
+            Triple t;
+            t.cin = (int32_t)cin;
+            t.cout = (int32_t)cout;
+            t.a = w;
+            stage[stage_index(g, n, cap)] = t;
+        }
         n++;
-        if (hist) atomicAdd(&hist[cin], 1);
+        atomicAdd(&hist[4 * cin + p], 1);
     }
 };
 
 struct UncountSink {
     int32_t* hist;
+    int32_t p;
     __device__ void emit(int64_t cin, int64_t, double)
     {
-        atomicAdd(&hist[cin], -1);
+        atomicAdd(&hist[4 * cin + p], -1);
     }
 };
 
@@ -55,15 +73,18 @@ struct BucketSink {
     uint32_t ordinal;
     const int64_t* __restrict__ bucket_off;
     int32_t* __restrict__ cursor;
-    uint64_t* __restrict__ key;
-    double* __restrict__ val;
+    KV* __restrict__ rec;
     int32_t* __restrict__ overflow;
-    __device__ void emit(int64_t cin, int64_t cout, double w)
+    int32_t p;
+    __device__ void emit(int64_t cin, int64_t cout, double a)
     {
-        int64_t pos = bucket_off[cin] + atomicAdd(&cursor[cin], 1);
-        if (pos < bucket_off[cin + 1]) {
-            key[pos] = ((uint64_t)cout << 32) | (uint64_t)ordinal;
-            val[pos] = w;
+        const int64_t b = 4 * cin + p;
+        int64_t pos = bucket_off[b] + atomicAdd(&cursor[b], 1);
+        if (pos < bucket_off[b + 1]) {
+            KV r;
+            r.key = ((uint64_t)cout << 32) | (uint64_t)ordinal;
+            r.val = a;
+            rec[pos] = r;
         } else {
             atomicAdd(overflow, 1);
         }
@@ -138,15 +159,40 @@ __global__ void k_bbox_partial(const double* __restrict__ x,
 __global__ void k_bbox_final(const double* __restrict__ part, int nparts,
                              double* __restrict__ bbox)
 {
-    if (threadIdx.x != 0) return;
     double v[4] = {part[0], part[1], part[2], part[3]};
-    for (int b = 1; b < nparts; b++) {
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
         v[0] = nan_min(v[0], part[4 * b + 0]);
         v[1] = nan_min(v[1], part[4 * b + 1]);
         v[2] = nan_max(v[2], part[4 * b + 2]);
         v[3] = nan_max(v[3], part[4 * b + 3]);
     }
-    for (int q = 0; q < 4; q++) bbox[q] = v[q];
+    __shared__ double sh[4][BBOX_BLOCK];
+    for (int q = 0; q < 4; q++) sh[q][threadIdx.x] = v[q];
+    __syncthreads();
+    for (int o = BBOX_BLOCK / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            sh[0][threadIdx.x] = nan_min(sh[0][threadIdx.x], sh[0][threadIdx.x + o]);
+            sh[1][threadIdx.x] = nan_min(sh[1][threadIdx.x], sh[1][threadIdx.x + o]);
+            sh[2][threadIdx.x] = nan_max(sh[2][threadIdx.x], sh[2][threadIdx.x + o]);
+            sh[3][threadIdx.x] = nan_max(sh[3][threadIdx.x], sh[3][threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 4; q++) bbox[q] = sh[q][0];
+}
+
+__global__ void k_interleave(const double* __restrict__ x,
+                             const double* __restrict__ y, int64_t n,
+                             double2* __restrict__ xy)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        double2 v;
+        v.x = x[i];
+        v.y = y[i];
+        xy[i] = v;
+    }
 }
 
 // Boundary polygon and boundary edge table (same length 2(nx-1)+2(ny-1)).
@@ -247,14 +293,11 @@ __global__ void k_row_start(Build B, int32_t* __restrict__ start,
 
 __global__ void k_chunk_start(Build B, int32_t* __restrict__ start)
 {
-    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (g >= B.nchunks) return;
-    const int p = pass_of_chunk(B, g);
-    const Pass& P = B.pass[p];
-    const int64_t local = g - P.chunk_base;
-    const int64_t row = local / P.cpr, c = local % P.cpr;
-    if (c == 0) return;
-    start[g] = chunk_start_guess(P, row, c * B.K);
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= B.nthreads) return;
+    const ChunkRef r = chunk_of_thread(B, t);
+    if (!r.valid || r.c == 0) return;
+    start[r.g] = chunk_start_guess(B.pass[r.p], r.row, r.c * B.K);
 }
 
 // ---- walks ---------------------------------------------------------------
@@ -271,26 +314,27 @@ __device__ inline void chunk_range(const Build& B, const Pass& P, int64_t c,
 __global__ void k_walk_count(Build B, const int32_t* __restrict__ start,
                              int32_t* __restrict__ end,
                              int32_t* __restrict__ count,
-                             int32_t* __restrict__ hist)
+                             int32_t* __restrict__ hist,
+                             Triple* __restrict__ stage, int64_t cap)
 {
-    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (g >= B.nchunks) return;
-    const int p = pass_of_chunk(B, g);
-    const Pass& P = B.pass[p];
-    const int64_t local = g - P.chunk_base;
-    const int64_t row = local / P.cpr, c = local % P.cpr;
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= B.nthreads) return;
+    const ChunkRef r = chunk_of_thread(B, t);
+    if (!r.valid) return;
+    const Pass& P = B.pass[r.p];
     int64_t k0, k1;
-    chunk_range(B, P, c, &k0, &k1);
-    CountSink sink{0, hist};
-    end[g] = walk_chunk(B, P, row, k0, k1, start[g], sink);
-    count[g] = sink.n;
+    chunk_range(B, P, r.c, &k0, &k1);
+    StageSink sink{0, t, cap, stage, hist, r.p};
+    end[r.g] = walk_chunk(B, P, r.row, k0, k1, start[r.g], sink);
+    count[r.g] = sink.n;
 }
 
 // One thread per row: make every chunk start where its predecessor ended.
 __global__ void k_fixup(Build B, int32_t* __restrict__ start,
                         int32_t* __restrict__ end, int32_t* __restrict__ count,
                         int32_t* __restrict__ hist, int32_t* __restrict__ nfix,
-                        int32_t* __restrict__ failed)
+                        int32_t* __restrict__ failed,
+                        Triple* __restrict__ stage, int64_t cap)
 {
     const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (r >= B.nrows_total) return;
@@ -306,9 +350,9 @@ __global__ void k_fixup(Build B, int32_t* __restrict__ start,
         if (start[g] == truth) continue;
         int64_t k0, k1;
         chunk_range(B, P, c, &k0, &k1);
-        UncountSink undo{hist};
+        UncountSink undo{hist, p};
         (void)walk_chunk(B, P, row, k0, k1, start[g], undo);
-        CountSink redo{0, hist};
+        StageSink redo{0, thread_of_chunk(P, row, c), cap, stage, hist, p};
         end[g] = walk_chunk(B, P, row, k0, k1, truth, redo);
         count[g] = redo.n;
         start[g] = truth;
@@ -322,79 +366,201 @@ __global__ void k_fixup(Build B, int32_t* __restrict__ start,
     if (fixes) atomicAdd(nfix, fixes);
 }
 
-__global__ void k_walk_emit(Build B, const int32_t* __restrict__ start,
-                            const int64_t* __restrict__ chunk_off,
-                            const int64_t* __restrict__ bucket_off,
-                            int32_t* __restrict__ cursor,
-                            uint64_t* __restrict__ key, double* __restrict__ val,
-                            int32_t* __restrict__ overflow)
+// Staged triples -> input-cell buckets, each tagged with its emission
+// ordinal (the chunk's first ordinal plus its position in the chunk).
+// Chunks whose triples overflowed the staging walk again, emitting straight
+// into the buckets.
+__global__ void k_scatter(Build B, const int32_t* __restrict__ start,
+                          const int32_t* __restrict__ count,
+                          const Triple* __restrict__ stage, int64_t cap,
+                          const int64_t* __restrict__ chunk_off,
+                          const int64_t* __restrict__ bucket_off,
+                          int32_t* __restrict__ cursor,
+                          KV* __restrict__ rec,
+                          int32_t* __restrict__ overflow,
+                          int32_t* __restrict__ restaged)
 {
-    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (g >= B.nchunks) return;
-    const int p = pass_of_chunk(B, g);
-    const Pass& P = B.pass[p];
-    const int64_t local = g - P.chunk_base;
-    const int64_t row = local / P.cpr, c = local % P.cpr;
-    int64_t k0, k1;
-    chunk_range(B, P, c, &k0, &k1);
-    BucketSink sink{(uint32_t)chunk_off[g], bucket_off, cursor, key, val,
-                    overflow};
-    (void)walk_chunk(B, P, row, k0, k1, start[g], sink);
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= B.nthreads) return;
+    const ChunkRef r = chunk_of_thread(B, t);
+    if (!r.valid) return;
+    const int32_t n = count[r.g];
+    const uint32_t ord = (uint32_t)chunk_off[r.g];
+    if (n > cap) {
+        const Pass& P = B.pass[r.p];
+        int64_t k0, k1;
+        chunk_range(B, P, r.c, &k0, &k1);
+        BucketSink sink{ord, bucket_off, cursor, rec, overflow, r.p};
+        (void)walk_chunk(B, P, r.row, k0, k1, start[r.g], sink);
+        atomicAdd(restaged, 1);
+        return;
+    }
+    for (int32_t q = 0; q < n; q++) {
+        const Triple e = stage[stage_index(t, q, cap)];
+        const int64_t bk = 4 * (int64_t)e.cin + r.p;
+        const int64_t pos = bucket_off[bk] + atomicAdd(&cursor[bk], 1);
+        if (pos < bucket_off[bk + 1]) {
+            KV v;
+            v.key = ((uint64_t)(uint32_t)e.cout << 32) | (uint64_t)(ord + (uint32_t)q);
+            v.val = e.a;
+            rec[pos] = v;
+        } else {
+            atomicAdd(overflow, 1);
+        }
+    }
 }
 
 // ---- reduce and CSR ------------------------------------------------------
 
-__global__ void k_bucket_reduce(const int64_t* __restrict__ bucket_off,
-                                uint64_t* __restrict__ key,
-                                double* __restrict__ val, int64_t n_in,
-                                int32_t* __restrict__ nnz_row,
-                                int32_t* __restrict__ long_list,
-                                int32_t* __restrict__ n_long)
+// Coalescing, one wavefront per group of 64 consecutive input cells (whose
+// buckets are one contiguous range).  The range is loaded into LDS with
+// coalesced loads, each lane sorts and reduces its own bucket there
+// (bucket_reduce), and the wave writes the group's unique (out, weight)
+// pairs back, packed from the start of the range, in row order.  A group
+// whose range exceeds the LDS share reduces in global memory instead and is
+// marked so the CSR pass copies it bucket by bucket.
+constexpr int RED_WAVES = 2;     // wavefronts per block
+constexpr int RED_CAP = 1536;    // LDS entries per wavefront (24 KiB)
+
+__global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
+    const int64_t* __restrict__ bucket_off, KV* __restrict__ rec,
+    const double* __restrict__ vol_in, const double* __restrict__ wi,
+    int64_t n_in, int32_t* __restrict__ nnz_row,
+    int32_t* __restrict__ group_mode, int32_t* __restrict__ long_list,
+    int32_t* __restrict__ n_long)
 {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= n_in) return;
-    const int64_t s = bucket_off[i], e = bucket_off[i + 1];
+    __shared__ uint64_t sk[RED_WAVES][RED_CAP];
+    __shared__ double sv[RED_WAVES][RED_CAP];
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t grp = (int64_t)blockIdx.x * RED_WAVES + wid;
+    const int64_t i0 = grp * 64;
+    if (i0 >= n_in) return;  // uniform per wavefront; no block barriers below
+    const int64_t i1 = i0 + 64 < n_in ? i0 + 64 : n_in;
+    const int64_t i = i0 + lane;
+    const int64_t base = bucket_off[4 * i0];
+    const int64_t total = bucket_off[4 * i1] - base;
+    int64_t s = 0, n = 0;
+    double vol = 1;
+    if (i < i1) {
+        s = bucket_off[4 * i] - base;
+        n = bucket_off[4 * i + 4] - bucket_off[4 * i];
+        vol = vol_in[i];
+    }
     bool lr = false;
-    nnz_row[i] = (int32_t)bucket_reduce(key + s, val + s, e - s, &lr);
-    if (lr) long_list[atomicAdd(n_long, 1)] = (int32_t)i;
+    if (total <= RED_CAP) {
+        uint64_t* K = sk[wid];
+        double* V = sv[wid];
+        for (int64_t q = lane; q < total; q += 64) {
+            const KV r = rec[base + q];
+            K[q] = r.key;
+            V[q] = r.val;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        int64_t m = 0;
+        if (i < i1) {
+            for (int64_t q = 0; q < n; q++) V[s + q] = weight_of(V[s + q], vol, wi, i);
+            m = bucket_reduce(K + s, V + s, n, &lr);
+        }
+        if (lr) m = n;  // left sorted, uncompacted, for k_bucket_long
+        const bool packed = __ballot(lr) == 0;
+        int64_t dst = s;  // a group with a long run keeps bucket starts
+        if (packed) {
+            // exclusive prefix of m over the wave: packed destination
+            int64_t incl = m;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                int64_t y = __shfl_up(incl, o, 64);
+                if (lane >= o) incl += y;
+            }
+            dst = incl - m;
+        }
+        for (int64_t t = 0; t < m; t++) {
+            KV r;
+            r.key = K[s + t];
+            r.val = V[s + t];
+            rec[base + dst + t] = r;
+        }
+        if (i < i1) {
+            nnz_row[i] = (int32_t)m;
+            if (lr) long_list[atomicAdd(n_long, 1)] = (int32_t)i;
+        }
+        if (lane == 0) group_mode[grp] = packed ? 0 : 1;
+    } else {
+        // oversized group: each lane finishes its bucket in global memory
+        if (i < i1) {
+            KV* r = rec + base + s;
+            for (int64_t q = 0; q < n; q++) r[q].val = weight_of(r[q].val, vol, wi, i);
+            sort_records(r, n);
+            nnz_row[i] = (int32_t)bucket_reduce_long(r, n);
+        }
+        if (lane == 0) group_mode[grp] = 1;
+    }
 }
 
-// Buckets with a run of more than 129 terms (see bucket_reduce); a
-// grid-stride loop over the (normally empty) list.
+// Buckets with a run of more than 129 terms (see bucket_reduce): left
+// sorted at their bucket start (their group is never packed) and finished
+// here.  Grid-stride over the normally empty list; fixes nnz_row.
 __global__ void k_bucket_long(const int64_t* __restrict__ bucket_off,
-                              uint64_t* __restrict__ key,
-                              double* __restrict__ val,
+                              KV* __restrict__ rec,
                               const int32_t* __restrict__ long_list,
-                              const int32_t* __restrict__ n_long)
+                              const int32_t* __restrict__ n_long,
+                              int32_t* __restrict__ nnz_row)
 {
     const int32_t nl = *n_long;
     for (int32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nl;
          q += gridDim.x * blockDim.x) {
         const int64_t i = long_list[q];
-        const int64_t s = bucket_off[i], e = bucket_off[i + 1];
-        (void)bucket_reduce_long(key + s, val + s, e - s);
+        const int64_t start = bucket_off[4 * i];
+        nnz_row[i] = (int32_t)bucket_reduce_long(rec + start,
+                                                 bucket_off[4 * i + 4] - start);
     }
 }
 
-__global__ void k_csr_in(const int64_t* __restrict__ bucket_off,
-                         const int64_t* __restrict__ row_off,
-                         const uint64_t* __restrict__ key,
-                         const double* __restrict__ val, int64_t n_in,
-                         int32_t* __restrict__ rowptr,
-                         int32_t* __restrict__ col, double* __restrict__ out_val,
-                         int32_t* __restrict__ hist_out)
+// CSR by input from the reduced groups: a packed group is one contiguous
+// copy; a fallback group copies bucket by bucket.  Also counts the entries
+// of every output cell for the transpose.
+__global__ void k_csr_groups(const int64_t* __restrict__ bucket_off,
+                             const int64_t* __restrict__ row_off,
+                             const int32_t* __restrict__ group_mode,
+                             const KV* __restrict__ rec, int64_t n_in,
+                             int32_t* __restrict__ rowptr,
+                             int32_t* __restrict__ col,
+                             double* __restrict__ out_val,
+                             int32_t* __restrict__ hist_out)
 {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i > n_in) return;
-    rowptr[i] = (int32_t)row_off[i];
-    if (i == n_in) return;
-    const int64_t s = bucket_off[i];
-    const int64_t d = row_off[i], m = row_off[i + 1] - d;
-    for (int64_t q = 0; q < m; q++) {
-        int32_t o = (int32_t)key[s + q];
-        col[d + q] = o;
-        out_val[d + q] = val[s + q];
-        atomicAdd(&hist_out[o], 1);
+    const int64_t grp = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    const int64_t i0 = grp * 64;
+    if (i0 > n_in) return;
+    if (i0 == n_in) {  // the closing row pointer
+        if (lane == 0) rowptr[n_in] = (int32_t)row_off[n_in];
+        return;
+    }
+    const int64_t i1 = i0 + 64 < n_in ? i0 + 64 : n_in;
+    if (i0 + lane < i1) rowptr[i0 + lane] = (int32_t)row_off[i0 + lane];
+    if (i1 == n_in && lane == 0) rowptr[n_in] = (int32_t)row_off[n_in];
+    const int64_t d0 = row_off[i0];
+    if (group_mode[grp] == 0) {
+        const int64_t b0 = bucket_off[4 * i0], cnt = row_off[i1] - d0;
+        for (int64_t q = lane; q < cnt; q += 64) {
+            const KV r = rec[b0 + q];
+            const int32_t o = (int32_t)r.key;
+            col[d0 + q] = o;
+            out_val[d0 + q] = r.val;
+            atomicAdd(&hist_out[o], 1);
+        }
+    } else if (i0 + lane < i1) {
+        const int64_t i = i0 + lane;
+        const int64_t sb = bucket_off[4 * i], d = row_off[i], m = row_off[i + 1] - d;
+        for (int64_t q = 0; q < m; q++) {
+            const KV r = rec[sb + q];
+            const int32_t o = (int32_t)r.key;
+            col[d + q] = o;
+            out_val[d + q] = r.val;
+            atomicAdd(&hist_out[o], 1);
+        }
     }
 }
 
@@ -422,16 +588,54 @@ __global__ void k_transpose_scatter(const int32_t* __restrict__ rowptr,
     }
 }
 
-__global__ void k_rows_sort(const int64_t* __restrict__ off, int64_t nrows,
-                            int32_t* __restrict__ col, double* __restrict__ val,
-                            int32_t* __restrict__ rowptr)
+// Orders every row of the transposed CSR by column (the rows were filled
+// in atomic order).  One wavefront per 64 consecutive rows, whose entries
+// are one contiguous range: loaded into LDS with coalesced loads, sorted
+// per lane in LDS, stored back coalesced.  Oversized groups sort in global
+// memory.  Also writes the int32 row pointers.
+constexpr int SORT_WAVES = 4;
+constexpr int SORT_CAP = 1536;  // entries per wavefront (18 KiB)
+
+__global__ __launch_bounds__(SORT_WAVES * 64) void k_rows_sort(
+    const int64_t* __restrict__ off, int64_t nrows, int32_t* __restrict__ col,
+    double* __restrict__ val, int32_t* __restrict__ rowptr)
 {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i > nrows) return;
-    rowptr[i] = (int32_t)off[i];
-    if (i == nrows) return;
-    const int64_t s = off[i];
-    sort_pairs(col + s, val + s, off[i + 1] - s);
+    __shared__ int32_t sc[SORT_WAVES][SORT_CAP];
+    __shared__ double sv[SORT_WAVES][SORT_CAP];
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t r0 = ((int64_t)blockIdx.x * SORT_WAVES + wid) * 64;
+    if (r0 > nrows) return;  // uniform per wavefront
+    const int64_t r = r0 + lane;
+    if (r <= nrows) rowptr[r] = (int32_t)off[r];
+    if (r0 == nrows) return;
+    const int64_t r1 = r0 + 64 < nrows ? r0 + 64 : nrows;
+    const int64_t base = off[r0], total = off[r1] - base;
+    int64_t s = 0, n = 0;
+    if (r < r1) {
+        s = off[r] - base;
+        n = off[r + 1] - off[r];
+    }
+    if (total <= SORT_CAP) {
+        int32_t* C = sc[wid];
+        double* V = sv[wid];
+        for (int64_t q = lane; q < total; q += 64) {
+            C[q] = col[base + q];
+            V[q] = val[base + q];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (r < r1) sort_pairs(C + s, V + s, n);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int64_t q = lane; q < total; q += 64) {
+            col[base + q] = C[q];
+            val[base + q] = V[q];
+        }
+    } else if (r < r1) {
+        sort_pairs(col + base + s, val + base + s, n);
+    }
 }
 
 __global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n)
@@ -446,6 +650,7 @@ __global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n)
 namespace {
 
 struct StaticTables {
+    DevBuf<double2> xy;
     DevBuf<double> bbox, part, bpx, bpy, be;
     DevBuf<int32_t> bcell;
     DevBuf<Box> bblk, bsup;
@@ -468,6 +673,7 @@ int prepare_static(const double* x, const double* y, int64_t nx, int64_t ny,
     const int64_t nblk = (nbe + BND_BLOCK - 1) / BND_BLOCK;
     const int64_t nsup = (nblk + BND_SUPER - 1) / BND_SUPER;
     int nparts = (int)std::min<int64_t>(grid_for(nv, BBOX_BLOCK), 1024);
+    RG_TRY(T.xy.alloc(nv, s));
     RG_TRY(T.bbox.alloc(4, s));
     RG_TRY(T.part.alloc(4 * (size_t)nparts, s));
     RG_TRY(T.bpx.alloc(nbe, s));
@@ -476,9 +682,11 @@ int prepare_static(const double* x, const double* y, int64_t nx, int64_t ny,
     RG_TRY(T.bcell.alloc(nbe, s));
     RG_TRY(T.bblk.alloc(nblk, s));
     RG_TRY(T.bsup.alloc(nsup, s));
+    k_interleave<<<std::min<unsigned>(grid_for(nv, 256), 16384), 256, 0, s>>>(
+        x, y, nv, T.xy.p);
     k_bbox_partial<<<nparts, BBOX_BLOCK, 0, s>>>(x, y, nv, T.part.p);
     RG_HIP(hipGetLastError());
-    k_bbox_final<<<1, 64, 0, s>>>(T.part.p, nparts, T.bbox.p);
+    k_bbox_final<<<1, BBOX_BLOCK, 0, s>>>(T.part.p, nparts, T.bbox.p);
     RG_HIP(hipGetLastError());
     k_boundary<<<grid_for(nbe, 256), 256, 0, s>>>(x, y, nx, ny, T.bpx.p,
                                                    T.bpy.p, T.be.p, T.bcell.p);
@@ -490,6 +698,7 @@ int prepare_static(const double* x, const double* y, int64_t nx, int64_t ny,
     RG_HIP(hipGetLastError());
     G.x = x;
     G.y = y;
+    G.xy = T.xy.p;
     G.nx = nx;
     G.ny = ny;
     G.bbox = T.bbox.p;
@@ -554,12 +763,13 @@ int build_conservative_2d(const double* xi, const double* yi, int64_t nxi,
     const int64_t n_in = (nxi - 1) * (nyi - 1), n_out = (nxo - 1) * (nyo - 1);
     if (n_in >= INT32_MAX || n_out >= INT32_MAX)
         return fail(RG_ERR_ARGUMENT, "more than 2^31 - 1 cells in one grid");
-    int64_t K = 16;
+    int64_t K = 16, stage_per_edge = 8;
     int block = 128;
     bool timings = false;
     if (tuning) {
         if (tuning->edges_per_lane > 0) K = tuning->edges_per_lane;
         if (tuning->block > 0) block = tuning->block;
+        if (tuning->stage_per_edge > 0) stage_per_edge = tuning->stage_per_edge;
         timings = (tuning->flags & RG_TUNE_TIMINGS) != 0;
     }
     if (block % 64 != 0 || block > 1024)
@@ -579,10 +789,8 @@ int build_conservative_2d(const double* xi, const double* yi, int64_t nxi,
     RG_TRY(prepare_static(xo, yo, nxo, nyo, tab_out, st_out, s));
 
     Build B;
-    B.vol_in = vol.p;
-    B.wi = wi;
     B.K = K;
-    int64_t chunk = 0, rows = 0;
+    int64_t chunk = 0, rows = 0, threads = 0;
     for (int p = 0; p < 4; p++) {
         Pass& P = B.pass[p];
         const int sweep_input = p >= 2;
@@ -604,14 +812,19 @@ int build_conservative_2d(const double* xi, const double* yi, int64_t nxi,
         P.st = sweep_input ? st_out : st_in;
         P.cpr = std::max<int64_t>(1, (P.nk - 1 + K - 1) / K);
         P.chunk_base = chunk;
+        P.thread_base = threads;
         B.row_base[p] = rows;
         chunk += P.nrows * P.cpr;
         rows += P.nrows;
+        threads += threads_of_pass(P.nrows, P.cpr);
     }
     B.nchunks = chunk;
     B.nrows_total = rows;
-    if (B.nchunks >= INT32_MAX)
+    B.nthreads = threads;
+    if (B.nthreads >= INT32_MAX)
         return fail(RG_ERR_ARGUMENT, "too many sweep chunks");
+    if (B.nthreads % 64 != 0)  // staging regions are whole wavefronts
+        return fail(RG_ERR_ARGUMENT, "walk thread count is not wavefront-aligned");
     tm.mark();  // 1
 
     // ---- start states
@@ -619,36 +832,41 @@ int build_conservative_2d(const double* xi, const double* yi, int64_t nxi,
     RG_TRY(start.alloc(B.nchunks, s));
     RG_TRY(end.alloc(B.nchunks, s));
     RG_TRY(count.alloc(B.nchunks, s));
-    RG_TRY(hist.alloc(n_in, s));
-    RG_TRY(counters.alloc(4, s));  // anomalies, fixups, failed, long runs
-    RG_TRY(fill_zero(hist.p, n_in, s));
-    RG_TRY(fill_zero(counters.p, 4, s));
+    RG_TRY(hist.alloc(4 * n_in, s));
+    // anomalies, fixups, failed/overflow, long runs, restaged chunks
+    RG_TRY(counters.alloc(8, s));
+    RG_TRY(fill_zero(hist.p, 4 * n_in, s));
+    RG_TRY(fill_zero(counters.p, 8, s));
     k_row_start<<<grid_for(B.nrows_total * 64, 256), 256, 0, s>>>(
         B, start.p, counters.p + 0);
     RG_HIP(hipGetLastError());
-    k_chunk_start<<<grid_for(B.nchunks, block), block, 0, s>>>(B, start.p);
+    k_chunk_start<<<grid_for(B.nthreads, block), block, 0, s>>>(B, start.p);
     RG_HIP(hipGetLastError());
     tm.mark();  // 2
 
-    // ---- count
-    k_walk_count<<<grid_for(B.nchunks, block), block, 0, s>>>(
-        B, start.p, end.p, count.p, hist.p);
+    // ---- count (staging the triples)
+    const int64_t cap = K * stage_per_edge;
+    DevBuf<Triple> stage;
+    RG_TRY(stage.alloc((size_t)B.nthreads * cap, s));
+    k_walk_count<<<grid_for(B.nthreads, block), block, 0, s>>>(
+        B, start.p, end.p, count.p, hist.p, stage.p, cap);
     RG_HIP(hipGetLastError());
     tm.mark();  // 3
 
     // ---- fix-up
     k_fixup<<<grid_for(B.nrows_total, 64), 64, 0, s>>>(
-        B, start.p, end.p, count.p, hist.p, counters.p + 1, counters.p + 2);
+        B, start.p, end.p, count.p, hist.p, counters.p + 1, counters.p + 2,
+        stage.p, cap);
     RG_HIP(hipGetLastError());
     DevBuf<int64_t> chunk_off, bucket_off;
     RG_TRY(chunk_off.alloc(B.nchunks + 1, s));
-    RG_TRY(bucket_off.alloc(n_in + 1, s));
+    RG_TRY(bucket_off.alloc(4 * n_in + 1, s));
     RG_TRY(scan_i32_to_i64(count.p, chunk_off.p, B.nchunks, s));
-    RG_TRY(scan_i32_to_i64(hist.p, bucket_off.p, n_in, s));
+    RG_TRY(scan_i32_to_i64(hist.p, bucket_off.p, 4 * n_in, s));
     int64_t tot[2];
     int32_t cnt[4];
     RG_TRY(read_scalars(chunk_off.p + B.nchunks, &tot[0], sizeof(int64_t), s));
-    RG_TRY(read_scalars(bucket_off.p + n_in, &tot[1], sizeof(int64_t), s));
+    RG_TRY(read_scalars(bucket_off.p + 4 * n_in, &tot[1], sizeof(int64_t), s));
     RG_TRY(read_scalars(counters.p, cnt, sizeof(cnt), s));
     tm.mark();  // 4
     if (cnt[2] != 0)
@@ -664,33 +882,35 @@ int build_conservative_2d(const double* xi, const double* yi, int64_t nxi,
         return fail(RG_ERR_ARGUMENT, "more than 2^32 triples in one element");
 
     // ---- emit
-    DevBuf<uint64_t> key;
-    DevBuf<double> val;
-    RG_TRY(key.alloc(T, s));
-    RG_TRY(val.alloc(T, s));
-    RG_TRY(fill_zero(hist.p, n_in, s));  // reused as the bucket cursor
-    k_walk_emit<<<grid_for(B.nchunks, block), block, 0, s>>>(
-        B, start.p, chunk_off.p, bucket_off.p, hist.p, key.p, val.p,
-        counters.p + 2);
+    DevBuf<KV> rec;
+    RG_TRY(rec.alloc(T, s));
+    RG_TRY(fill_zero(hist.p, 4 * n_in, s));  // reused as the bucket cursor
+    k_scatter<<<grid_for(B.nthreads, block), block, 0, s>>>(
+        B, start.p, count.p, stage.p, cap, chunk_off.p, bucket_off.p, hist.p,
+        rec.p, counters.p + 2, counters.p + 4);
     RG_HIP(hipGetLastError());
+    stage.reset();
     tm.mark();  // 5
 
     // ---- reduce -> CSR by input
-    DevBuf<int32_t> nnz_row, long_list;
+    DevBuf<int32_t> nnz_row, long_list, group_mode;
     DevBuf<int64_t> row_off;
+    const int64_t n_groups = (n_in + 63) / 64;
     RG_TRY(nnz_row.alloc(n_in, s));
     RG_TRY(long_list.alloc(n_in, s));
+    RG_TRY(group_mode.alloc(n_groups, s));
     RG_TRY(row_off.alloc(n_in + 1, s));
-    k_bucket_reduce<<<grid_for(n_in, 128), 128, 0, s>>>(
-        bucket_off.p, key.p, val.p, n_in, nnz_row.p, long_list.p,
-        counters.p + 3);
-    k_bucket_long<<<4, 64, 0, s>>>(bucket_off.p, key.p, val.p, long_list.p,
-                                   counters.p + 3);
+    k_reduce_groups<<<(unsigned)((n_groups + RED_WAVES - 1) / RED_WAVES),
+                      RED_WAVES * 64, 0, s>>>(bucket_off.p, rec.p, vol.p, wi,
+                                              n_in, nnz_row.p, group_mode.p,
+                                              long_list.p, counters.p + 3);
+    k_bucket_long<<<4, 64, 0, s>>>(bucket_off.p, rec.p, long_list.p,
+                                   counters.p + 3, nnz_row.p);
     RG_HIP(hipGetLastError());
     RG_TRY(scan_i32_to_i64(nnz_row.p, row_off.p, n_in, s));
     int64_t nnz = 0;
     RG_TRY(read_scalars(row_off.p + n_in, &nnz, sizeof(int64_t), s));
-    int32_t cnt2[4];
+    int32_t cnt2[8];
     RG_TRY(read_scalars(counters.p, cnt2, sizeof(cnt2), s));
     if (cnt2[2] != 0)
         return fail(RG_ERR_HIP, "internal: emission overflowed its buckets (" +
@@ -702,6 +922,10 @@ int build_conservative_2d(const double* xi, const double* yi, int64_t nxi,
     W->n_in = n_in;
     W->n_out = n_out;
     W->nnz = nnz;
+    W->shape_in[0] = nxi - 1;
+    W->shape_in[1] = nyi - 1;
+    W->shape_out[0] = nxo - 1;
+    W->shape_out[1] = nyo - 1;
     (void)hipGetDevice(&W->device);
     DevBuf<int32_t> rp_in, c_in, rp_out, c_out, hist_out;
     DevBuf<double> v_in, v_out;
@@ -717,11 +941,11 @@ int build_conservative_2d(const double* xi, const double* yi, int64_t nxi,
         if ((st = hist_out.alloc(n_out, s))) break;
         if ((st = off_out.alloc(n_out + 1, s))) break;
         if ((st = fill_zero(hist_out.p, n_out, s))) break;
-        k_csr_in<<<grid_for(n_in + 1, 128), 128, 0, s>>>(
-            bucket_off.p, row_off.p, key.p, val.p, n_in, rp_in.p, c_in.p,
-            v_in.p, hist_out.p);
+        k_csr_groups<<<grid_for((n_groups + 1) * 64, 256), 256, 0, s>>>(
+            bucket_off.p, row_off.p, group_mode.p, rec.p, n_in,
+            rp_in.p, c_in.p, v_in.p, hist_out.p);
         if (hipGetLastError() != hipSuccess) {
-            st = fail(RG_ERR_HIP, "k_csr_in launch failed");
+            st = fail(RG_ERR_HIP, "k_csr_groups launch failed");
             break;
         }
         tm.mark();  // 6
@@ -731,8 +955,10 @@ int build_conservative_2d(const double* xi, const double* yi, int64_t nxi,
         k_transpose_scatter<<<grid_for(n_in, 128), 128, 0, s>>>(
             rp_in.p, c_in.p, v_in.p, n_in, off_out.p, hist_out.p, c_out.p,
             v_out.p);
-        k_rows_sort<<<grid_for(n_out + 1, 128), 128, 0, s>>>(
-            off_out.p, n_out, c_out.p, v_out.p, rp_out.p);
+        const int64_t sort_groups = n_out / 64 + 1;
+        k_rows_sort<<<(unsigned)((sort_groups + SORT_WAVES - 1) / SORT_WAVES),
+                      SORT_WAVES * 64, 0, s>>>(off_out.p, n_out, c_out.p,
+                                               v_out.p, rp_out.p);
         if (hipGetLastError() != hipSuccess) {
             st = fail(RG_ERR_HIP, "transpose launch failed");
             break;
@@ -754,6 +980,7 @@ int build_conservative_2d(const double* xi, const double* yi, int64_t nxi,
     S.n_chunks = B.nchunks;
     S.n_fixups = cnt[1];
     S.n_search_anomalies = cnt[0];
+    S.n_restaged = cnt2[4];
     // every inside sub-segment emits two triples except on the first and
     // last row of a sweep, which emit one
     S.n_subsegments = T / 2;
@@ -783,6 +1010,10 @@ void weights_release(rg_weights* W)
     dev_free(W->val_out);
     dev_free(W->val_in_f32);
     dev_free(W->val_out_f32);
+    for (auto& pl : W->plan) {
+        dev_free(pl.box);
+        dev_free(pl.off);
+    }
     delete W;
 }
 
@@ -936,6 +1167,8 @@ int import_triples(const int64_t* ii, const int64_t* io, const double* w,
     W->n_in = n_in;
     W->n_out = n_out;
     W->nnz = nnz;
+    W->shape_in[1] = n_in;
+    W->shape_out[1] = n_out;
     (void)hipGetDevice(&W->device);
     int st = group_csr(io, ii, w, nnz, n_out, n_in, &W->rowptr_out, &W->col_out,
                        &W->val_out, s);
@@ -1013,6 +1246,19 @@ int rg_weights_import(const int64_t* indices_input,
     if (!out) return fail(RG_ERR_ARGUMENT, "out is NULL");
     return import_triples(indices_input, indices_output, values, nnz, n_in,
                           n_out, out, (hipStream_t)stream);
+}
+
+int rg_weights_set_shape(rg_weights_t w, int64_t ncx_in, int64_t ncy_in,
+                         int64_t ncx_out, int64_t ncy_out)
+{
+    if (!w) return fail(RG_ERR_ARGUMENT, "weights handle is NULL");
+    if (ncx_in * ncy_in != w->n_in || ncx_out * ncy_out != w->n_out)
+        return fail(RG_ERR_ARGUMENT, "shapes do not match the cell counts");
+    w->shape_in[0] = ncx_in;
+    w->shape_in[1] = ncy_in;
+    w->shape_out[0] = ncx_out;
+    w->shape_out[1] = ncy_out;
+    return RG_OK;
 }
 
 int rg_weights_free(rg_weights_t w)

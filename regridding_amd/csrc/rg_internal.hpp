@@ -104,6 +104,9 @@ inline unsigned grid_for(int64_t n, int block)
 struct rg_weights {
     int device = 0;
     int64_t n_in = 0, n_out = 0, nnz = 0;
+    // 2-D cell shapes of the two grids (rows of the CSRs are flat cells);
+    // (1, n) when unknown (imported weights without rg_weights_set_shape)
+    int64_t shape_in[2] = {1, 0}, shape_out[2] = {1, 0};
     int32_t* rowptr_in = nullptr;   // n_in + 1
     int32_t* col_in = nullptr;      // nnz, output cell of each entry
     double* val_in = nullptr;       // nnz
@@ -112,5 +115,13 @@ struct rg_weights {
     double* val_out = nullptr;      // nnz
     float* val_in_f32 = nullptr;    // lazily made single-precision copies
     float* val_out_f32 = nullptr;
+    // apply plans (see rg_apply.hip), one per direction, made on first use
+    struct Plan {
+        int64_t tile_w = 0;        // tile shape the plan was made for
+        int32_t* box = nullptr;    // per tile: i0, j0, rows, cols of the patch
+        uint16_t* off = nullptr;   // per CSR entry: offset inside its patch
+        int64_t ntiles = 0;
+        int64_t max_cells = 0;     // largest patch (cells)
+    } plan[2];
     rg_build_stats stats = {};
 };

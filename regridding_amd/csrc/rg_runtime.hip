@@ -288,6 +288,14 @@ int scan_i64(const int64_t* in, int64_t* out, int64_t n, hipStream_t s)
     return scan_impl<int64_t>(in, out, n, s);
 }
 
+__global__ void k_copy_f64(const double* __restrict__ a, double* __restrict__ b,
+                           int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
 }  // namespace rg
 
 // ---- exported plumbing ---------------------------------------------------
@@ -391,6 +399,46 @@ int rg_stream_sync(void* stream)
 int rg_device_sync(void)
 {
     RG_HIP(hipDeviceSynchronize());
+    return RG_OK;
+}
+
+int rg_copy_f64(const double* src, double* dst, int64_t n, void* stream)
+{
+    if (n <= 0) return RG_OK;
+    if (!src || !dst) return fail(RG_ERR_ARGUMENT, "NULL argument");
+    unsigned g = grid_for(n, 256);
+    if (g > 16384) g = 16384;
+    k_copy_f64<<<g, 256, 0, (hipStream_t)stream>>>(src, dst, n);
+    RG_HIP(hipGetLastError());
+    return RG_OK;
+}
+
+int rg_event_create(void** event)
+{
+    if (!event) return fail(RG_ERR_ARGUMENT, "event is NULL");
+    hipEvent_t e = nullptr;
+    RG_HIP(hipEventCreate(&e));
+    *event = (void*)e;
+    return RG_OK;
+}
+
+int rg_event_destroy(void* event)
+{
+    if (event) RG_HIP(hipEventDestroy((hipEvent_t)event));
+    return RG_OK;
+}
+
+int rg_event_record(void* event, void* stream)
+{
+    RG_HIP(hipEventRecord((hipEvent_t)event, (hipStream_t)stream));
+    return RG_OK;
+}
+
+int rg_event_elapsed_ms(void* start, void* stop, float* ms)
+{
+    if (!ms) return fail(RG_ERR_ARGUMENT, "ms is NULL");
+    RG_HIP(hipEventSynchronize((hipEvent_t)stop));
+    RG_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
     return RG_OK;
 }
 

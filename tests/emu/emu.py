@@ -35,7 +35,7 @@ def lib():
         L = ctypes.CDLL(_PATH)
         L.rge_build.restype = _i64
         L.rge_build.argtypes = [
-            _pd, _pd, _i64, _i64, _pd, _pd, _i64, _i64, _pd, _i64, ctypes.c_int,
+            _pd, _pd, _i64, _i64, _pd, _pd, _i64, _i64, _pd, _i64, _i64, ctypes.c_int,
             ctypes.POINTER(_pi), ctypes.POINTER(_pi), ctypes.POINTER(_pd), _pi,
             ctypes.POINTER(_pi), ctypes.POINTER(_pi), ctypes.POINTER(_pd), _pi,
         ]
@@ -61,7 +61,8 @@ def _take(p, n, dtype):
     return np.ctypeslib.as_array(p, shape=(n,)).copy()
 
 
-def build_2d(grid_input, grid_output, weights_input=None, K: int = 16, guess: int = 0):
+def build_2d(grid_input, grid_output, weights_input=None, K: int = 16, guess: int = 0,
+             stage_per_edge: int = 8):
     """
     Emulated GPU build.  Returns ``(flat, coalesced, stats)``: the
     uncoalesced triples in emission order, the coalesced triples sorted by
@@ -74,11 +75,11 @@ def build_2d(grid_input, grid_output, weights_input=None, K: int = 16, guess: in
     wi = None if weights_input is None else np.ascontiguousarray(weights_input, np.float64)
     fi, fo, fw, ci, co, cw = _pi(), _pi(), _pd(), _pi(), _pi(), _pd()
     nf = _i64()
-    stats = np.zeros(5, np.int64)
+    stats = np.zeros(7, np.int64)
     L = lib()
     nnz = L.rge_build(
         _d(xi), _d(yi), xi.shape[0], xi.shape[1], _d(xo), _d(yo), xo.shape[0], xo.shape[1],
-        None if wi is None else _d(wi), K, guess,
+        None if wi is None else _d(wi), K, stage_per_edge, guess,
         ctypes.byref(fi), ctypes.byref(fo), ctypes.byref(fw), ctypes.byref(nf),
         ctypes.byref(ci), ctypes.byref(co), ctypes.byref(cw),
         stats.ctypes.data_as(_pi),
@@ -88,7 +89,7 @@ def build_2d(grid_input, grid_output, weights_input=None, K: int = 16, guess: in
     coal = (_take(ci, nnz, np.int64), _take(co, nnz, np.int64), _take(cw, nnz, np.float64))
     for p in (fi, fo, fw, ci, co, cw):
         L.rge_free(ctypes.cast(p, ctypes.c_void_p))
-    keys = ("chunks", "fixups", "anomalies", "long", "failed")
+    keys = ("chunks", "fixups", "anomalies", "long", "failed", "restaged", "map_errors")
     return flat, coal, dict(zip(keys, stats.tolist()))
 
 

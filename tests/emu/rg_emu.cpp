@@ -19,19 +19,30 @@ using namespace rg;
 
 namespace {
 
-struct HCount {
+struct HStage {
     int32_t n;
+    int64_t g, cap;
+    Triple* stage;
     int32_t* hist;
-    __host__ __device__ void emit(int64_t cin, int64_t, double)
+    int32_t p;
+    __host__ __device__ void emit(int64_t cin, int64_t cout, double w)
     {
+        if (n < cap) {
+            Triple t;
+            t.cin = (int32_t)cin;
+            t.cout = (int32_t)cout;
+            t.a = w;
+            stage[stage_index(g, n, cap)] = t;
+        }
         n++;
-        if (hist) hist[cin]++;
+        hist[4 * cin + p]++;
     }
 };
 
 struct HUncount {
     int32_t* hist;
-    __host__ __device__ void emit(int64_t cin, int64_t, double) { hist[cin]--; }
+    int32_t p;
+    __host__ __device__ void emit(int64_t cin, int64_t, double) { hist[4 * cin + p]--; }
 };
 
 struct HFlat {
@@ -39,11 +50,13 @@ struct HFlat {
     int64_t* in;
     int64_t* out;
     double* w;
+    const double* vol;
+    const double* wi;
     __host__ __device__ void emit(int64_t cin, int64_t cout, double v)
     {
         in[pos] = cin;
         out[pos] = cout;
-        w[pos] = v;
+        w[pos] = weight_of(v, vol[cin], wi, cin);
         pos++;
     }
 };
@@ -54,9 +67,10 @@ struct HBucket {
     int32_t* cursor;
     uint64_t* key;
     double* val;
+    int32_t pass;
     __host__ __device__ void emit(int64_t cin, int64_t cout, double v)
     {
-        int64_t p = off[cin] + cursor[cin]++;
+        int64_t p = off[4 * cin + pass] + cursor[4 * cin + pass]++;
         key[p] = ((uint64_t)cout << 32) | ordinal;
         val[p] = v;
         ordinal++;
@@ -73,6 +87,7 @@ double nmax(double a, double b)
 }
 
 struct Tables {
+    std::vector<double2> xy;
     std::vector<double> bbox, bpx, bpy, be;
     std::vector<int32_t> bcell;
     std::vector<Box> bblk, bsup;
@@ -82,6 +97,11 @@ void prepare(const double* x, const double* y, int64_t nx, int64_t ny,
              Tables& T, StaticGrid& G)
 {
     int64_t nv = nx * ny, nbe = 2 * (nx - 1) + 2 * (ny - 1);
+    T.xy.resize(nv);
+    for (int64_t i = 0; i < nv; i++) {
+        T.xy[i].x = x[i];
+        T.xy[i].y = y[i];
+    }
     T.bbox = {x[0], y[0], x[0], y[0]};
     for (int64_t i = 0; i < nv; i++) {
         T.bbox[0] = nmin(T.bbox[0], x[i]);
@@ -126,7 +146,7 @@ void prepare(const double* x, const double* y, int64_t nx, int64_t ny,
         }
         T.bsup[s] = r;
     }
-    G.x = x; G.y = y; G.nx = nx; G.ny = ny;
+    G.x = x; G.y = y; G.xy = T.xy.data(); G.nx = nx; G.ny = ny;
     G.bbox = T.bbox.data();
     G.bpx = T.bpx.data(); G.bpy = T.bpy.data(); G.nbp = nbe;
     G.be = T.be.data(); G.bcell = T.bcell.data(); G.nbe = nbe;
@@ -182,13 +202,15 @@ extern "C" {
 // Emulated build.  Outputs (malloc'd, free with rge_free):
 //   flat_*   uncoalesced triples in emission-ordinal order (n_flat)
 //   csr_*    coalesced triples sorted by (in, out) (n_csr)
-// stats: [n_chunks, n_fixups, n_anomalies, n_long_buckets, failed]
+// stats: [n_chunks, n_fixups, n_anomalies, n_long_buckets, failed, restaged,
+//         thread/chunk map errors]
 // `guess` selects the chunk start guesses: 0 = the GPU's cell-search guess,
 // 1 = always "outside" (every chunk of a row that starts inside must then be
 // repaired by the fix-up pass), 2 = pseudo-random garbage states.
 int64_t rge_build(const double* xi, const double* yi, int64_t nxi, int64_t nyi,
                   const double* xo, const double* yo, int64_t nxo, int64_t nyo,
-                  const double* wi, int64_t K, int guess, int64_t** flat_in,
+                  const double* wi, int64_t K, int64_t stage_per_edge, int guess,
+                  int64_t** flat_in,
                   int64_t** flat_out, double** flat_w, int64_t* n_flat,
                   int64_t** csr_in, int64_t** csr_out, double** csr_w,
                   int64_t* stats)
@@ -203,10 +225,8 @@ int64_t rge_build(const double* xi, const double* yi, int64_t nxi, int64_t nyi,
     prepare(xo, yo, nxo, nyo, To, Go);
 
     Build B;
-    B.vol_in = vol.data();
-    B.wi = wi;
     B.K = K;
-    int64_t chunk = 0, rows = 0;
+    int64_t chunk = 0, rows = 0, threads = 0;
     for (int p = 0; p < 4; p++) {
         Pass& P = B.pass[p];
         const int sweep_input = p >= 2, axis = p & 1;
@@ -221,15 +241,34 @@ int64_t rge_build(const double* xi, const double* yi, int64_t nxi, int64_t nyi,
         P.st = sweep_input ? Go : Gi;
         P.cpr = std::max<int64_t>(1, (P.nk - 1 + K - 1) / K);
         P.chunk_base = chunk;
+        P.thread_base = threads;
         B.row_base[p] = rows;
         chunk += P.nrows * P.cpr;
         rows += P.nrows;
+        threads += threads_of_pass(P.nrows, P.cpr);
     }
     B.nchunks = chunk;
     B.nrows_total = rows;
+    B.nthreads = threads;
+    // the thread <-> chunk maps: every chunk owned by exactly one thread
+    int64_t map_errors = 0;
+    {
+        std::vector<int32_t> owner(chunk, 0);
+        for (int64_t t = 0; t < threads; t++) {
+            ChunkRef r = chunk_of_thread(B, t);
+            if (!r.valid) continue;
+            if (r.g < 0 || r.g >= chunk) { map_errors++; continue; }
+            owner[r.g]++;
+            if (thread_of_chunk(B.pass[r.p], r.row, r.c) != t) map_errors++;
+        }
+        for (int64_t g = 0; g < chunk; g++)
+            if (owner[g] != 1) map_errors++;
+    }
 
-    std::vector<int32_t> start(chunk), end(chunk), count(chunk), hist(n_in, 0);
-    int64_t anomalies = 0, fixups = 0, failed = 0, nlong = 0;
+    std::vector<int32_t> start(chunk), end(chunk), count(chunk), hist(4 * n_in, 0);
+    const int64_t cap = K * stage_per_edge;
+    std::vector<Triple> stage((size_t)threads * cap);
+    int64_t anomalies = 0, fixups = 0, failed = 0, nlong = 0, restaged = 0;
     uint64_t lcg = 0x9E3779B97F4A7C15ull;
     for (int64_t r = 0; r < rows; r++) {
         int p = pass_of_row(B, r);
@@ -259,7 +298,7 @@ int64_t rge_build(const double* xi, const double* yi, int64_t nxi, int64_t nyi,
         int64_t local = g - P.chunk_base, row = local / P.cpr, c = local % P.cpr;
         int64_t k0, k1;
         chunk_range(B, P, c, &k0, &k1);
-        HCount s{0, hist.data()};
+        HStage s{0, thread_of_chunk(P, row, c), cap, stage.data(), hist.data(), p};
         end[g] = walk_chunk(B, P, row, k0, k1, start[g], s);
         count[g] = s.n;
     }
@@ -275,9 +314,9 @@ int64_t rge_build(const double* xi, const double* yi, int64_t nxi, int64_t nyi,
             if (start[g] == truth) continue;
             int64_t k0, k1;
             chunk_range(B, P, c, &k0, &k1);
-            HUncount u{hist.data()};
+            HUncount u{hist.data(), p};
             (void)walk_chunk(B, P, row, k0, k1, start[g], u);
-            HCount rc{0, hist.data()};
+            HStage rc{0, thread_of_chunk(P, row, c), cap, stage.data(), hist.data(), p};
             end[g] = walk_chunk(B, P, row, k0, k1, truth, rc);
             count[g] = rc.n;
             start[g] = truth;
@@ -289,9 +328,9 @@ int64_t rge_build(const double* xi, const double* yi, int64_t nxi, int64_t nyi,
                 break;
             }
     }
-    std::vector<int64_t> choff(chunk + 1, 0), boff(n_in + 1, 0);
+    std::vector<int64_t> choff(chunk + 1, 0), boff(4 * n_in + 1, 0);
     for (int64_t g = 0; g < chunk; g++) choff[g + 1] = choff[g] + count[g];
-    for (int64_t i = 0; i < n_in; i++) boff[i + 1] = boff[i] + hist[i];
+    for (int64_t i = 0; i < 4 * n_in; i++) boff[i + 1] = boff[i] + hist[i];
     const int64_t T = choff[chunk];
     // flat emission in ordinal order (for comparison with the oracle sweep)
     int64_t* fi = (int64_t*)malloc(sizeof(int64_t) * (T ? T : 1));
@@ -299,28 +338,45 @@ int64_t rge_build(const double* xi, const double* yi, int64_t nxi, int64_t nyi,
     double* fw = (double*)malloc(sizeof(double) * (T ? T : 1));
     std::vector<uint64_t> key(T ? T : 1);
     std::vector<double> val(T ? T : 1);
-    std::vector<int32_t> cursor(n_in, 0);
+    std::vector<int32_t> cursor(4 * n_in, 0);
     for (int64_t g = 0; g < chunk; g++) {
         int p = pass_of_chunk(B, g);
         const Pass& P = B.pass[p];
         int64_t local = g - P.chunk_base, row = local / P.cpr, c = local % P.cpr;
         int64_t k0, k1;
         chunk_range(B, P, c, &k0, &k1);
-        HFlat f{choff[g], fi, fo, fw};
+        HFlat f{choff[g], fi, fo, fw, vol.data(), wi};
         (void)walk_chunk(B, P, row, k0, k1, start[g], f);
-        HBucket hb{(uint32_t)choff[g], boff.data(), cursor.data(), key.data(),
-                   val.data()};
-        (void)walk_chunk(B, P, row, k0, k1, start[g], hb);
+        if (count[g] > cap) {  // k_scatter: overflowed chunks walk again
+            HBucket hb{(uint32_t)choff[g], boff.data(), cursor.data(),
+                       key.data(), val.data(), p};
+            (void)walk_chunk(B, P, row, k0, k1, start[g], hb);
+            restaged++;
+            continue;
+        }
+        for (int32_t t = 0; t < count[g]; t++) {
+            const Triple e = stage[stage_index(thread_of_chunk(P, row, c), t, cap)];
+            int64_t pos = boff[4 * e.cin + p] + cursor[4 * e.cin + p]++;
+            key[pos] = ((uint64_t)(uint32_t)e.cout << 32) | (uint64_t)(choff[g] + t);
+            val[pos] = e.a;
+        }
     }
     // reduce
     std::vector<int64_t> nnz_row(n_in);
     for (int64_t i = 0; i < n_in; i++) {
         bool lr = false;
-        int64_t s = boff[i], n = boff[i + 1] - s;
+        int64_t s = boff[4 * i], n = boff[4 * i + 4] - s;
+        for (int64_t q = 0; q < n; q++) val[s + q] = weight_of(val[s + q], vol[i], wi, i);
         nnz_row[i] = bucket_reduce(key.data() + s, val.data() + s, n, &lr);
         if (lr) {
             nlong++;
-            (void)bucket_reduce_long(key.data() + s, val.data() + s, n);
+            std::vector<KV> r(n);
+            for (int64_t q = 0; q < n; q++) r[q] = KV{key[s + q], val[s + q]};
+            nnz_row[i] = bucket_reduce_long(r.data(), n);
+            for (int64_t q = 0; q < nnz_row[i]; q++) {
+                key[s + q] = r[q].key;
+                val[s + q] = r[q].val;
+            }
         }
     }
     int64_t nnz = 0;
@@ -332,8 +388,8 @@ int64_t rge_build(const double* xi, const double* yi, int64_t nxi, int64_t nyi,
     for (int64_t i = 0; i < n_in; i++)
         for (int64_t a = 0; a < nnz_row[i]; a++) {
             ci[q] = i;
-            co[q] = (int64_t)key[boff[i] + a];
-            cw[q] = val[boff[i] + a];
+            co[q] = (int64_t)key[boff[4 * i] + a];
+            cw[q] = val[boff[4 * i] + a];
             q++;
         }
     *flat_in = fi;
@@ -348,6 +404,8 @@ int64_t rge_build(const double* xi, const double* yi, int64_t nxi, int64_t nyi,
     stats[2] = anomalies;
     stats[3] = nlong;
     stats[4] = failed;
+    stats[5] = restaged;
+    stats[6] = map_errors;
     (void)n_out;
     return nnz;
 }

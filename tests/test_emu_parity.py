@@ -44,7 +44,7 @@ def test_chunked_walk_reproduces_serial_walk(name, K, guess):
         assert np.array_equal(a, b)
     for a, b in zip(coal, coal_ref):
         assert np.array_equal(a, b)
-    assert st["failed"] == 0 and st["anomalies"] == 0
+    assert st["failed"] == 0 and st["anomalies"] == 0 and st["map_errors"] == 0
     if guess == 0 and K < 1000:
         # the cell-search guesses are right in generic position
         assert st["fixups"] == 0
@@ -65,6 +65,17 @@ def test_unperturbed_degenerate_grids():
             assert st["failed"] == 0
             for a, b in zip(coal, ref):
                 assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("stage_per_edge", [1, 3])
+def test_staging_overflow_rewalks(stage_per_edge):
+    # chunks emitting more triples than their staging slots walk a second time
+    ci, co = _grids("coarse_to_fine")
+    ref = oracle.coalesce(*oracle.sweep_2d(ci, co))
+    _, coal, st = emu.build_2d(ci, co, K=8, stage_per_edge=stage_per_edge)
+    assert st["restaged"] > 0
+    for a, b in zip(coal, ref):
+        assert np.array_equal(a, b)
 
 
 def test_weights_input():

@@ -64,6 +64,17 @@ def test_build_tuning_invariant(K, block):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("stage_per_edge", [1, 2, 32])
+def test_build_staging_overflow(stage_per_edge):
+    ci, co = _perturbed(*CASES["coarse_to_fine"]())
+    h = build_2d(ci, co, edges_per_lane=8, stage_per_edge=stage_per_edge)
+    got = h.export()
+    for a, b in zip(got, oracle.weights_2d(ci, co)):
+        assert np.array_equal(a, b)
+    if stage_per_edge == 1:
+        assert h.stats()["n_restaged"] > 0
+
+
 def test_build_weights_input():
     ci, co = _perturbed(*CASES["coarse_to_fine"]())
     wi = np.random.default_rng(7).uniform(0.5, 3.0, (ci[0].shape[0] - 1, ci[0].shape[1] - 1))
@@ -124,6 +135,26 @@ def test_apply_matches_oracle(D):
     # adjoint: the reference applied to the swapped triples (transpose_weights)
     y = np.random.default_rng(D + 1).normal(size=(D, h.n_out))
     assert np.array_equal(_apply(h, y, transpose=True), oracle.apply(io, ii, w, y, h.n_in))
+
+
+@pytest.mark.parametrize("tile_w", [-16, 8, 16, 32, 256])
+@pytest.mark.parametrize("ipp", [1, 4, 16])
+def test_apply_launch_shapes_same_bits(tile_w, ipp):
+    # planned (LDS-staged) and direct gathers, any tile and image grouping:
+    # the per-output accumulation order never changes
+    ci, co = _perturbed(*CASES["coarse_to_fine"]())
+    h = build_2d(ci, co)
+    ii, io, w = h.export()
+    x = np.random.default_rng(4).normal(size=(7, h.n_in))
+    y = np.random.default_rng(5).normal(size=(7, h.n_out))
+    ref = oracle.apply(ii, io, w, x, h.n_out)
+    ref_t = oracle.apply(io, ii, w, y, h.n_in)
+    try:
+        check(lib().rg_set_apply_tuning(tile_w, ipp))
+        assert np.array_equal(_apply(h, x), ref)
+        assert np.array_equal(_apply(h, y, transpose=True), ref_t)
+    finally:
+        check(lib().rg_set_apply_tuning(16, 8))
 
 
 def test_apply_f32_tolerance():
