@@ -5,14 +5,18 @@
 //             boundary edge table in the outside-scan order, edge boxes
 //   start     chunk 0 of every row: the reference's row start (wavefront per
 //             row); chunk c > 0: a guessed start from the cell search
-//   count     every chunk walks, counts its triples, records its end state
-//             and adds its triples to the per-input-cell histogram
+//   count     every chunk walks, counts its triples, records its end state,
+//             adds its triples to the (input cell, pass) histogram and stages
+//             up to `cap` (cin, cout, term) triples per walk thread
 //   fixup     per row: chunks whose start differs from the predecessor's end
-//             are walked again from the true state (histogram corrected)
+//             are walked again from the true state (histogram corrected,
+//             staging rewritten)
 //   emit      scans give each chunk its first emission ordinal and each
-//             input cell its bucket; every chunk walks again and writes
-//             (out << 32 | ordinal, w) straight into the buckets
-//   reduce    per input cell: sort the bucket, reduce duplicate pairs in
+//             (cell, pass) its bucket; staged triples are scattered into the
+//             buckets as (out << 32 | ordinal, term) records; chunks whose
+//             triples overflowed the staging walk a third time
+//   reduce    per group of 64 input cells (in LDS): divide by the input cell
+//             area, sort each bucket, reduce duplicate pairs in
 //             np.add.reduceat order -> CSR by input (the reference order)
 //   transpose CSR by output (columns ascending) for the forward apply
 #include <hip/hip_runtime.h>
@@ -34,63 +38,124 @@ namespace rg {
 // the reduction) while the triples one pass writes for neighbouring cells
 // land close together.
 
+// Histogram increments are combined in registers: a walk's consecutive
+// triples mostly fall in one or two (cell, pass) buckets (the static cell of
+// a sweep over the output grid; the left and right cells of an input edge),
+// so two pending (bucket, count) entries replace most atomics.
+struct HistCache {
+    int64_t b0 = -1, b1 = -1;
+    int32_t n0 = 0, n1 = 0;
+    __device__ void add(int32_t* __restrict__ hist, int64_t b, int32_t d)
+    {
+        if (b == b0) {
+            n0 += d;
+        } else if (b == b1) {
+            n1 += d;
+        } else {
+            if (b1 >= 0 && n1 != 0) atomicAdd(&hist[b1], n1);
+            b1 = b0;
+            n1 = n0;
+            b0 = b;
+            n0 = d;
+        }
+    }
+    __device__ void flush(int32_t* __restrict__ hist)
+    {
+        if (b0 >= 0 && n0 != 0) atomicAdd(&hist[b0], n0);
+        if (b1 >= 0 && n1 != 0) atomicAdd(&hist[b1], n1);
+        b0 = b1 = -1;
+        n0 = n1 = 0;
+    }
+};
+
 // Counting walk: counts the chunk's triples, adds them to the per-input-cell
-// histogram and keeps the first `cap` of them in the staging buffer.
+// histogram and keeps the first `cap` of them in the staging buffer.  Call
+// flush() after the walk.
 struct StageSink {
     int32_t n;
     int64_t g, cap;
     Triple* __restrict__ stage;
     int32_t* __restrict__ hist;
     int32_t p;
-    __device__ void emit(int64_t cin, int64_t cout, double w)
+    HistCache hc;
+    __device__ void emit(int64_t cin, int64_t cout, double a)
     {
         if (n < cap) {
             Triple t;
             t.cin = (int32_t)cin;
             t.cout = (int32_t)cout;
-            t.a = w;
+            t.a = a;
             stage[stage_index(g, n, cap)] = t;
         }
         n++;
-        atomicAdd(&hist[4 * cin + p], 1);
+        hc.add(hist, 4 * cin + p, 1);
     }
+    __device__ void flush() { hc.flush(hist); }
 };
 
 struct UncountSink {
     int32_t* hist;
     int32_t p;
+    HistCache hc;
     __device__ void emit(int64_t cin, int64_t, double)
     {
-        atomicAdd(&hist[4 * cin + p], -1);
+        hc.add(hist, 4 * cin + p, -1);
     }
+    __device__ void flush() { hc.flush(hist); }
 };
 
-// Writes each triple into its input cell's bucket.  The emit walk repeats
-// the (fixed-up) count walk, so a bucket never receives more than its
-// histogram count; the bound check only turns a broken invariant into a
-// reported error instead of an out-of-bounds store.
+// Bucket cursors: cursor[b] starts at the first record of bucket b and each
+// store claims the next one with one returning atomic (record positions fit
+// in 32 bits: a build holds fewer than 2^32 triples).  A store is only
+// bounds-checked against the record count; k_cursor_check then verifies that
+// every cursor ended exactly at its bucket's end, which holds iff each
+// bucket received exactly the triples the count walk histogrammed.
+__device__ inline void put_records(uint32_t* __restrict__ cursor,
+                                   KV* __restrict__ rec, uint32_t n_rec,
+                                   int32_t* __restrict__ overflow, int64_t b,
+                                   int64_t cout, uint32_t ordinal, double a)
+{
+    const uint32_t pos = atomicAdd(&cursor[b], 1u);
+    if (pos < n_rec) {
+        KV r;
+        r.key = ((uint64_t)(uint32_t)cout << 32) | (uint64_t)ordinal;
+        r.val = a;
+        rec[pos] = r;
+    } else {
+        atomicAdd(overflow, 1);
+    }
+}
+
+// Writes each triple into its input cell's bucket (chunks whose triples
+// overflowed the staging buffer walk again with this sink).
 struct BucketSink {
     uint32_t ordinal;
-    const int64_t* __restrict__ bucket_off;
-    int32_t* __restrict__ cursor;
+    uint32_t* __restrict__ cursor;
     KV* __restrict__ rec;
+    uint32_t n_rec;
     int32_t* __restrict__ overflow;
     int32_t p;
     __device__ void emit(int64_t cin, int64_t cout, double a)
     {
-        const int64_t b = 4 * cin + p;
-        int64_t pos = bucket_off[b] + atomicAdd(&cursor[b], 1);
-        if (pos < bucket_off[b + 1]) {
-            KV r;
-            r.key = ((uint64_t)cout << 32) | (uint64_t)ordinal;
-            r.val = a;
-            rec[pos] = r;
-        } else {
-            atomicAdd(overflow, 1);
-        }
+        put_records(cursor, rec, n_rec, overflow, 4 * cin + p, cout, ordinal, a);
         ordinal++;
     }
 };
+
+__global__ void k_cursor_init(const int64_t* __restrict__ bucket_off,
+                              uint32_t* __restrict__ cursor, int64_t nb)
+{
+    const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (b < nb) cursor[b] = (uint32_t)bucket_off[b];
+}
+
+__global__ void k_cursor_check(const int64_t* __restrict__ bucket_off,
+                               const uint32_t* __restrict__ cursor, int64_t nb,
+                               int32_t* __restrict__ bad)
+{
+    const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (b < nb && (int64_t)cursor[b] != bucket_off[b + 1]) atomicAdd(bad, 1);
+}
 
 // ---- prepare -------------------------------------------------------------
 
@@ -324,8 +389,9 @@ __global__ void k_walk_count(Build B, const int32_t* __restrict__ start,
     const Pass& P = B.pass[r.p];
     int64_t k0, k1;
     chunk_range(B, P, r.c, &k0, &k1);
-    StageSink sink{0, t, cap, stage, hist, r.p};
+    StageSink sink{0, t, cap, stage, hist, r.p, {}};
     end[r.g] = walk_chunk(B, P, r.row, k0, k1, start[r.g], sink);
+    sink.flush();
     count[r.g] = sink.n;
 }
 
@@ -350,10 +416,12 @@ __global__ void k_fixup(Build B, int32_t* __restrict__ start,
         if (start[g] == truth) continue;
         int64_t k0, k1;
         chunk_range(B, P, c, &k0, &k1);
-        UncountSink undo{hist, p};
+        UncountSink undo{hist, p, {}};
         (void)walk_chunk(B, P, row, k0, k1, start[g], undo);
-        StageSink redo{0, thread_of_chunk(P, row, c), cap, stage, hist, p};
+        undo.flush();
+        StageSink redo{0, thread_of_chunk(P, row, c), cap, stage, hist, p, {}};
         end[g] = walk_chunk(B, P, row, k0, k1, truth, redo);
+        redo.flush();
         count[g] = redo.n;
         start[g] = truth;
         fixes++;
@@ -374,9 +442,8 @@ __global__ void k_scatter(Build B, const int32_t* __restrict__ start,
                           const int32_t* __restrict__ count,
                           const Triple* __restrict__ stage, int64_t cap,
                           const int64_t* __restrict__ chunk_off,
-                          const int64_t* __restrict__ bucket_off,
-                          int32_t* __restrict__ cursor,
-                          KV* __restrict__ rec,
+                          uint32_t* __restrict__ cursor,
+                          KV* __restrict__ rec, uint32_t n_rec,
                           int32_t* __restrict__ overflow,
                           int32_t* __restrict__ restaged)
 {
@@ -390,23 +457,34 @@ __global__ void k_scatter(Build B, const int32_t* __restrict__ start,
         const Pass& P = B.pass[r.p];
         int64_t k0, k1;
         chunk_range(B, P, r.c, &k0, &k1);
-        BucketSink sink{ord, bucket_off, cursor, rec, overflow, r.p};
+        BucketSink sink{ord, cursor, rec, n_rec, overflow, r.p};
         (void)walk_chunk(B, P, r.row, k0, k1, start[r.g], sink);
         atomicAdd(restaged, 1);
         return;
     }
-    for (int32_t q = 0; q < n; q++) {
-        const Triple e = stage[stage_index(t, q, cap)];
-        const int64_t bk = 4 * (int64_t)e.cin + r.p;
-        const int64_t pos = bucket_off[bk] + atomicAdd(&cursor[bk], 1);
-        if (pos < bucket_off[bk + 1]) {
-            KV v;
-            v.key = ((uint64_t)(uint32_t)e.cout << 32) | (uint64_t)(ord + (uint32_t)q);
-            v.val = e.a;
-            rec[pos] = v;
+    // consecutive staged triples of one bucket (a walk inside one static
+    // cell of a sweep over the output grid) claim their records with one
+    // atomic
+    int32_t q0 = 0;
+    while (q0 < n) {
+        const Triple e0 = stage[stage_index(t, q0, cap)];
+        int32_t q1 = q0 + 1;
+        while (q1 < n && q1 - q0 < 16 && stage[stage_index(t, q1, cap)].cin == e0.cin)
+            q1++;
+        const uint32_t m = (uint32_t)(q1 - q0);
+        const uint32_t pos = atomicAdd(&cursor[4 * (int64_t)e0.cin + r.p], m);
+        if ((uint64_t)pos + m <= n_rec) {
+            for (int32_t q = q0; q < q1; q++) {
+                const Triple e = q == q0 ? e0 : stage[stage_index(t, q, cap)];
+                KV v;
+                v.key = ((uint64_t)(uint32_t)e.cout << 32) | (uint64_t)(ord + (uint32_t)q);
+                v.val = e.a;
+                rec[pos + (uint32_t)(q - q0)] = v;
+            }
         } else {
             atomicAdd(overflow, 1);
         }
+        q0 = q1;
     }
 }
 
@@ -884,10 +962,15 @@ int build_conservative_2d(const double* xi, const double* yi, int64_t nxi,
     // ---- emit
     DevBuf<KV> rec;
     RG_TRY(rec.alloc(T, s));
-    RG_TRY(fill_zero(hist.p, 4 * n_in, s));  // reused as the bucket cursor
+    uint32_t* cursor = (uint32_t*)hist.p;  // reused: one word per bucket
+    k_cursor_init<<<grid_for(4 * n_in, 256), 256, 0, s>>>(bucket_off.p, cursor,
+                                                          4 * n_in);
     k_scatter<<<grid_for(B.nthreads, block), block, 0, s>>>(
-        B, start.p, count.p, stage.p, cap, chunk_off.p, bucket_off.p, hist.p,
-        rec.p, counters.p + 2, counters.p + 4);
+        B, start.p, count.p, stage.p, cap, chunk_off.p, cursor, rec.p,
+        (uint32_t)T, counters.p + 2, counters.p + 4);
+    RG_HIP(hipGetLastError());
+    k_cursor_check<<<grid_for(4 * n_in, 256), 256, 0, s>>>(bucket_off.p, cursor,
+                                                           4 * n_in, counters.p + 2);
     RG_HIP(hipGetLastError());
     stage.reset();
     tm.mark();  // 5
@@ -913,8 +996,9 @@ int build_conservative_2d(const double* xi, const double* yi, int64_t nxi,
     int32_t cnt2[8];
     RG_TRY(read_scalars(counters.p, cnt2, sizeof(cnt2), s));
     if (cnt2[2] != 0)
-        return fail(RG_ERR_HIP, "internal: emission overflowed its buckets (" +
-                                    std::to_string(cnt2[2]) + " triples)");
+        return fail(RG_ERR_HIP, "internal: the emission did not match the "
+                                "counted buckets (" + std::to_string(cnt2[2]) +
+                                    " mismatches)");
     if (nnz >= INT32_MAX)
         return fail(RG_ERR_ARGUMENT, "more than 2^31 - 1 weights in one element");
 
