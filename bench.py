@@ -167,16 +167,20 @@ def main() -> int:
 
     sweep = []
     if args.apply_sweep:
-        for tw in (-32, -16, 64, 32, 16, 8):
-            for ipp in (16, 8, 4):
-                check(L.rg_set_apply_tuning(tw, ipp))
-                ms = time_apply(L.rg_apply_f64, dx64, dy64, 0)
-                ms_t = time_apply(L.rg_apply_f64, dy64, dt64, 1)
-                sweep.append({"tile_w": tw, "images_per_pass": ipp, "ms": round(ms, 5),
-                              "adjoint_ms": round(ms_t, 5)})
-                log(f"apply sweep tile_w={tw} images/pass={ipp}: {ms:.4f} ms, adjoint {ms_t:.4f} ms")
+        shapes = [(tw, ipp, kib) for tw in (-16, 64, 32, 16, 8) for ipp in (16, 8, 4)
+                  for kib in (32, 64)]
+        for tw, ipp, kib in shapes:
+            if tw < 0 and kib == 64:
+                continue
+            check(L.rg_set_apply_tuning(tw, ipp, kib))
+            ms = time_apply(L.rg_apply_f64, dx64, dy64, 0)
+            ms_t = time_apply(L.rg_apply_f64, dy64, dt64, 1)
+            sweep.append({"tile_w": tw, "images_per_pass": ipp, "lds_kib": kib,
+                          "ms": round(ms, 5), "adjoint_ms": round(ms_t, 5)})
+            log(f"apply sweep tile_w={tw} images/pass={ipp} lds={kib}KiB: "
+                f"{ms:.4f} ms, adjoint {ms_t:.4f} ms")
         best = min(sweep, key=lambda r: r["ms"])
-        check(L.rg_set_apply_tuning(best["tile_w"], best["images_per_pass"]))
+        check(L.rg_set_apply_tuning(best["tile_w"], best["images_per_pass"], best["lds_kib"]))
         log(f"apply sweep best: {best}")
 
     ms_apply64 = time_apply(L.rg_apply_f64, dx64, dy64, 0)

@@ -91,6 +91,19 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_apply(
 // row load per box row, and gathers from LDS.
 
 constexpr int LDS_BYTES = 64 * 1024;  // staged patch per block
+constexpr int STAGE_UNROLL = 8;
+
+// a / b from a float reciprocal and one correction step (the float quotient
+// is off by at most one).  Exact for 0 <= a <= 65536 and 1 <= b <= 16384,
+// checked exhaustively; a staged patch holds at most 16384 elements.
+__device__ inline int32_t fast_div(int32_t a, int32_t b, float inv_b)
+{
+    int32_t q = (int32_t)((float)a * inv_b);
+    const int32_t r = a - q * b;
+    if (r < 0) q--;
+    else if (r >= b) q++;
+    return q;
+}
 
 __global__ __launch_bounds__(APPLY_BLOCK) void k_plan(
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
@@ -186,16 +199,30 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_apply_planned(
     const int64_t nd = batch - d0 < DB ? batch - d0 : DB;
     const bool staged = cells > 0 && cells * nd * (int64_t)sizeof(T) <= lds_bytes;
     if (staged) {
-        // box row (image m, row ii) per wavefront; lanes along the row
-        const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        const int32_t ncell = (int32_t)cells;
-        for (int32_t m = 0; m < (int32_t)nd; m++) {
-            const T* src_m = x + (d0 + m) * ncols + (int64_t)i0 * src_gy + j0;
-            T* dst_m = patch + m * ncell;
-            for (int32_t ii = wid; ii < rows; ii += APPLY_BLOCK / 64) {
-                const T* src = src_m + (int64_t)ii * src_gy;
-                T* dst = dst_m + ii * cols;
-                for (int32_t jj = lane; jj < cols; jj += 64) dst[jj] = src[jj];
+        // The box of nd images, flattened (image, row, column) over the
+        // block's lanes; each lane issues STAGE_UNROLL loads before its LDS
+        // stores, so the staging costs about one memory latency instead of
+        // one per box row.
+        const int32_t ncell = (int32_t)cells, total = ncell * (int32_t)nd;
+        const float inv_cell = 1.0f / (float)ncell, inv_cols = 1.0f / (float)cols;
+        const T* xb = x + d0 * ncols + (int64_t)i0 * src_gy + j0;
+        for (int32_t e0 = threadIdx.x; e0 < total; e0 += APPLY_BLOCK * STAGE_UNROLL) {
+            T v[STAGE_UNROLL];
+#pragma unroll
+            for (int u = 0; u < STAGE_UNROLL; u++) {
+                const int32_t e = e0 + u * APPLY_BLOCK;
+                if (e < total) {
+                    const int32_t m = fast_div(e, ncell, inv_cell);
+                    const int32_t rem = e - m * ncell;
+                    const int32_t ii = fast_div(rem, cols, inv_cols);
+                    const int32_t jj = rem - ii * cols;
+                    v[u] = xb[(int64_t)m * ncols + (int64_t)ii * src_gy + jj];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < STAGE_UNROLL; u++) {
+                const int32_t e = e0 + u * APPLY_BLOCK;
+                if (e < total) patch[e] = v[u];
             }
         }
         __syncthreads();
@@ -212,7 +239,24 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_apply_planned(
     for (int m = 0; m < DB; m++) acc[m] = T(0);
     if (staged) {
         const int32_t ncell = (int32_t)cells;
-        for (int32_t q = s; q < e; q++) {
+        int32_t q = s;
+        // four entries' loads in flight at once; the terms are still added
+        // one at a time in column order
+        for (; q + 4 <= e; q += 4) {
+            int32_t o[4];
+            T wq[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                o[u] = off[q + u];
+                wq[u] = w[q + u];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int m = 0; m < DB; m++)
+                    if (m < nd) acc[m] = acc[m] + wq[u] * patch[m * ncell + o[u]];
+        }
+        for (; q < e; q++) {
             const int32_t o = off[q];
             const T wq = w[q];
 #pragma unroll
@@ -247,6 +291,7 @@ namespace {
 
 int g_tile_w = 16;          // tuned on the plain 2049^2 -> 1537^2 pair
 int g_images_per_pass = 8;
+int64_t g_lds_limit = LDS_BYTES;
 bool g_use_plan = true;
 std::mutex g_plan_mu;
 
@@ -312,9 +357,9 @@ int launch_apply(rg_weights* W, int dir, const int32_t* rowptr,
     const int64_t nb = (G.ntiles + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
     int db = g_images_per_pass;
     while (db > 1 && db > batch) db >>= 1;
-    // staged patches: keep the LDS per block at or under 32 KiB (5+ blocks
-    // per CU) by staging fewer images per pass
-    while (P && db > 1 && P->max_cells * db * (int64_t)sizeof(T) > LDS_BYTES / 2) db >>= 1;
+    // staged patches: stage fewer images per pass rather than fall back to
+    // the direct gather for the largest tiles
+    while (P && db > 1 && P->max_cells * db * (int64_t)sizeof(T) > g_lds_limit) db >>= 1;
     dim3 g((unsigned)nb, (unsigned)((batch + db - 1) / db)), bl(APPLY_BLOCK);
     if (P) {
         int64_t lds = P->max_cells * db * (int64_t)sizeof(T);
@@ -418,7 +463,7 @@ int rg_apply_f32(rg_weights_t w, int transpose, const float* values_input,
                                w->shape_out, w->shape_in[1], batch, s);
 }
 
-int rg_set_apply_tuning(int tile_w, int images_per_pass)
+int rg_set_apply_tuning(int tile_w, int images_per_pass, int lds_kib)
 {
     // negative tile_w: keep the tile width, turn the staged (planned) apply
     // off; the direct-gather kernel computes the same bits
@@ -438,6 +483,11 @@ int rg_set_apply_tuning(int tile_w, int images_per_pass)
             images_per_pass != 8 && images_per_pass != 16)
             return fail(RG_ERR_ARGUMENT, "images_per_pass must be 1, 2, 4, 8 or 16");
         g_images_per_pass = images_per_pass;
+    }
+    if (lds_kib) {
+        if (lds_kib < 1 || lds_kib > LDS_BYTES / 1024)
+            return fail(RG_ERR_ARGUMENT, "lds_kib must be in 1..64");
+        g_lds_limit = (int64_t)lds_kib * 1024;
     }
     return RG_OK;
 }

@@ -138,8 +138,8 @@ def test_apply_matches_oracle(D):
 
 
 @pytest.mark.parametrize("tile_w", [-16, 8, 16, 32, 256])
-@pytest.mark.parametrize("ipp", [1, 4, 16])
-def test_apply_launch_shapes_same_bits(tile_w, ipp):
+@pytest.mark.parametrize("ipp,lds_kib", [(1, 32), (4, 1), (16, 64)])
+def test_apply_launch_shapes_same_bits(tile_w, ipp, lds_kib):
     # planned (LDS-staged) and direct gathers, any tile and image grouping:
     # the per-output accumulation order never changes
     ci, co = _perturbed(*CASES["coarse_to_fine"]())
@@ -150,11 +150,11 @@ def test_apply_launch_shapes_same_bits(tile_w, ipp):
     ref = oracle.apply(ii, io, w, x, h.n_out)
     ref_t = oracle.apply(io, ii, w, y, h.n_in)
     try:
-        check(lib().rg_set_apply_tuning(tile_w, ipp))
+        check(lib().rg_set_apply_tuning(tile_w, ipp, lds_kib))
         assert np.array_equal(_apply(h, x), ref)
         assert np.array_equal(_apply(h, y, transpose=True), ref_t)
     finally:
-        check(lib().rg_set_apply_tuning(16, 8))
+        check(lib().rg_set_apply_tuning(16, 8, 64))
 
 
 def test_apply_f32_tolerance():
