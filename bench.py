@@ -54,6 +54,9 @@ def parse_args():
     p.add_argument("--block", type=int, default=0)
     p.add_argument("--apply-sweep", action="store_true",
                    help="time the apply over tile widths x images per pass")
+    p.add_argument("--no-batched", action="store_true",
+                   help="skip the batched-vs-plain comparison over 256 mixed pairs")
+    p.add_argument("--batched-pairs", type=int, default=256)
     return p.parse_args()
 
 
@@ -101,7 +104,18 @@ def main() -> int:
     dy32 = DeviceArray.empty((B, n_out), np.float32)
     dt64 = DeviceArray.empty((B, n_in), np.float64)
     check(L.rg_device_sync())
-    tune = dict(edges_per_lane=args.edges_per_lane, block=args.block)
+    tuned = None
+    if args.edges_per_lane or args.block:
+        tune = dict(edges_per_lane=args.edges_per_lane, block=args.block)
+    else:
+        # the library's autotuner picks the launch shape for this size class
+        from regridding_amd import _autotune
+        from regridding_amd._conservative import pack_pairs
+
+        tuned = _autotune.tune(pack_pairs([(ci, co)]), reps=2)
+        tune = dict(edges_per_lane=tuned["edges_per_lane"], block=tuned["block"])
+        log(f"autotuned build: edges_per_lane={tune['edges_per_lane']} block={tune['block']} "
+            f"({tuned['ms']:.3f} ms)")
 
     def build(timings=False):
         return build_2d_device(xi, yi, ci[0].shape, xo, yo, co[0].shape, None,
@@ -215,7 +229,8 @@ def main() -> int:
             "images": B,
             "nnz": nnz,
             "triples": st["n_triples"],
-            "edges_per_lane": args.edges_per_lane or 16,
+            "edges_per_lane": tune["edges_per_lane"] or 16,
+            "block": tune["block"] or 128,
         },
         "roofline": {
             "kernel": "k_apply<double,16> (CSR gather apply, FP64, 16 images)",
@@ -229,6 +244,7 @@ def main() -> int:
             "ms_per_launch": round(ms_apply64, 5),
         },
         "build_ms": round(build_ms, 4),
+        **({"build_autotune": tuned} if tuned else {}),
         "build_phases_ms": {k[3:]: round(v, 4) for k, v in st.items() if k.startswith("ms_")},
         "build_fixups": st["n_fixups"],
         "build_restaged": st["n_restaged"],
@@ -240,10 +256,129 @@ def main() -> int:
         **({"apply_sweep": sweep} if sweep else {}),
     }
 
+    h.free()
+    del dx32, dy32, dt64
+    if not args.no_batched:
+        result["batched"] = bench_batched(args)
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(ci, co, values, h)
+        result["cpu_baseline"] = cpu_baseline(ci, co, values, build())
     print(json.dumps(result), flush=True)
     return 0
+
+
+def bench_batched(args) -> dict:
+    """Many uneven grid pairs (per-axis vertex counts log-uniform in
+    [33, 1025], 1-8 FP64 value arrays each), build + apply, two ways on the
+    same device-resident inputs: the plain path (one rg_build_conservative_2d
+    and one rg_apply_f64 call per pair, in sequence) and the batched path
+    (one rg_build_conservative_2d_batched and one rg_apply_batched_f64 call
+    for all pairs).  Both produce the same bits; the ratio of their
+    grid-pairs/s is reported."""
+    import ctypes
+
+    from regridding_amd import _util, workloads
+    from regridding_amd._conservative import build_2d_batched, pack_pairs
+    from regridding_amd._lib import DeviceArray, EventTimer, Weights, check, lib, rg_apply_job
+
+    L = lib()
+    n_pairs = args.batched_pairs
+    t0 = time.perf_counter()
+    raw = workloads.batched_pairs(n_pairs=n_pairs)
+    items, vals = [], []
+    for p in raw:
+        ci, co, *_ = _util.normalize_coordinates(p["grid_input"], p["grid_output"], perturb=True)
+        items.append((ci, co))
+        vals.append(np.ascontiguousarray(p["values"].reshape(p["values"].shape[0], -1)))
+    del raw
+    packed = pack_pairs(items)
+    n_in = [(ci[0].shape[0] - 1) * (ci[0].shape[1] - 1) for ci, _ in items]
+    n_out = [(co[0].shape[0] - 1) * (co[0].shape[1] - 1) for _, co in items]
+    nb = [v.shape[0] for v in vals]
+    x_off = np.concatenate([[0], np.cumsum([v.size for v in vals])])
+    y_off = np.concatenate([[0], np.cumsum([b * n for b, n in zip(nb, n_out)])])
+    dX = DeviceArray.from_host(np.concatenate([v.ravel() for v in vals]))
+    dYp = DeviceArray.empty((int(y_off[-1]),), np.float64)
+    dYb = DeviceArray.empty((int(y_off[-1]),), np.float64)
+    del vals
+    check(L.rg_device_sync())
+    log(f"batched workload: {n_pairs} pairs, {sum(n_in)} input cells, {sum(n_out)} output "
+        f"cells, {sum(nb)} value arrays (prepared in {time.perf_counter() - t0:.1f} s)")
+
+    def plain():
+        for k in range(n_pairs):
+            d = packed.desc[k]
+            h = ctypes.c_void_p()
+            check(L.rg_build_conservative_2d(d.xi, d.yi, d.nxi, d.nyi, d.xo, d.yo, d.nxo, d.nyo,
+                                             None, None, ctypes.byref(h), 0))
+            check(L.rg_apply_f64(h, 0, dX.ptr + 8 * int(x_off[k]), dYp.ptr + 8 * int(y_off[k]),
+                                 nb[k], 0))
+            check(L.rg_weights_free(h))
+
+    jobs = (rg_apply_job * n_pairs)()
+    from regridding_amd import _autotune
+
+    t1 = time.perf_counter()
+    tuned = _autotune.tune(packed, reps=1)
+    log(f"autotuned batch build: edges_per_lane={tuned['edges_per_lane']} block={tuned['block']} "
+        f"({tuned['ms']:.2f} ms; tuning took {time.perf_counter() - t1:.1f} s)")
+
+    def batched(timings=False):
+        hs = build_2d_batched(packed, timings=timings, edges_per_lane=tuned["edges_per_lane"],
+                              block=tuned["block"])
+        for k, h in enumerate(hs):
+            j = jobs[k]
+            j.weights, j.transpose, j.batch = h.ptr, 0, nb[k]
+            j.values_input = dX.ptr + 8 * int(x_off[k])
+            j.values_output = dYb.ptr + 8 * int(y_off[k])
+        check(L.rg_apply_batched_f64(jobs, n_pairs, 0))
+        return hs
+
+    def timed(fn, reps):
+        fn()
+        check(L.rg_device_sync())
+        t = EventTimer(0)
+        t.start()
+        for _ in range(reps):
+            r = fn()
+            if r is not None:
+                for h in r:
+                    h.free()
+        ms = t.stop() / reps
+        check(L.rg_device_sync())
+        return ms
+
+    ms_plain = timed(plain, 2)
+    ms_batched = timed(batched, 3)
+    hs = batched(timings=True)
+    st = hs[0].stats()
+    triples = sum(h.stats()["n_triples"] for h in hs)
+    nnz = sum(h.nnz for h in hs)
+    for h in hs:
+        h.free()
+    same = bool(np.array_equal(dYp.to_host(), dYb.to_host()))
+    ratio = ms_plain / ms_batched
+    log(f"batched: plain {ms_plain:.2f} ms ({n_pairs / ms_plain * 1e3:.0f} pairs/s), batched "
+        f"{ms_batched:.2f} ms ({n_pairs / ms_batched * 1e3:.0f} pairs/s), x{ratio:.2f}; "
+        f"outputs identical: {same}; batch phases "
+        + " ".join(f"{k[3:]}={v:.2f}" for k, v in st.items() if k.startswith("ms_")))
+    return {
+        "workload": f"{n_pairs} independent pairs, per-axis vertices log-uniform in [33,1025], "
+                    "spacing ratio U[0.5,2], rotations U(0,pi/4), 1-8 FP64 Poisson arrays each",
+        "pairs": n_pairs,
+        "input_cells": int(sum(n_in)),
+        "output_cells": int(sum(n_out)),
+        "value_arrays": int(sum(nb)),
+        "triples": int(triples),
+        "nnz": int(nnz),
+        "plain_ms": round(ms_plain, 3),
+        "plain_pairs_per_s": round(n_pairs / ms_plain * 1e3, 1),
+        "batched_ms": round(ms_batched, 3),
+        "batched_pairs_per_s": round(n_pairs / ms_batched * 1e3, 1),
+        "batched_vs_plain": round(ratio, 3),
+        "outputs_bit_identical": same,
+        "batch_build_phases_ms": {k[3:]: round(v, 3) for k, v in st.items() if k.startswith("ms_")},
+        "batch_autotune": tuned,
+    }
 
 
 def cpu_baseline(ci, co, values, handle) -> dict:

@@ -110,6 +110,45 @@ int rg_build_conservative_2d(const double* xi, const double* yi, int64_t nxi,
                              const rg_tuning* tuning, rg_weights_t* out,
                              void* stream);
 
+/* One grid pair of a batched build: the arguments of
+ * rg_build_conservative_2d (device pointers). */
+typedef struct rg_pair_desc {
+    const double* xi;
+    const double* yi;
+    int64_t nxi, nyi;
+    const double* xo;
+    const double* yo;
+    int64_t nxo, nyo;
+    const double* wi; /* (nxi-1)*(nyi-1) input weighting, or NULL */
+} rg_pair_desc;
+
+/* Replaces the per-element loop of
+ * _weights/_weights_conservative.py:122-139, which calls
+ * weights_conservative_2d (and then _coalesce) once per orthogonal element:
+ * builds npairs independent grid pairs, of any mix of sizes, together --
+ * every phase is one launch over the concatenated work of all pairs.
+ * out[k] receives the handle of pairs[k], identical (bit for bit) to what
+ * rg_build_conservative_2d returns for that pair alone.  The handles share
+ * device buffers, released with the last of them (rg_weights_free each).
+ * In rg_weights_stats of a batched handle, n_triples and n_chunks are the
+ * pair's; the other counters and the phase times are the batch's. */
+int rg_build_conservative_2d_batched(const rg_pair_desc* pairs, int64_t npairs,
+                                     const rg_tuning* tuning, rg_weights_t* out,
+                                     void* stream);
+
+/* Autotuner of the (batched) build; no reference counterpart.  Builds
+ * `pairs` once per candidate (edges_per_lane[a], block[b]) -- reps timed
+ * runs each after one untimed run, HIP events on `stream` -- and returns
+ * the fastest setting in *best (stage_per_edge and flags from `base`, which
+ * may be NULL) and its median time in *best_ms.  times, if not NULL,
+ * receives the n_epl x n_block median times (ms), row-major.  Results of
+ * a build do not depend on the tuning. */
+int rg_tune_build(const rg_pair_desc* pairs, int64_t npairs,
+                  const int32_t* edges_per_lane, int n_epl,
+                  const int32_t* block, int n_block, int reps,
+                  const rg_tuning* base, rg_tuning* best, double* best_ms,
+                  double* times, void* stream);
+
 /* Sizes and build counters of a handle. */
 int rg_weights_info(rg_weights_t w, int64_t* n_in, int64_t* n_out,
                     int64_t* nnz);
@@ -161,6 +200,24 @@ int rg_apply_f64(rg_weights_t w, int transpose, const double* values_input,
  * counterpart (the reference always accumulates in float64). */
 int rg_apply_f32(rg_weights_t w, int transpose, const float* values_input,
                  float* values_output, int64_t batch, void* stream);
+
+/* One apply of a batched apply call: as the arguments of rg_apply_f64 /
+ * rg_apply_f32 (values are float64 for the f64 call, float32 for f32). */
+typedef struct rg_apply_job {
+    rg_weights_t weights;
+    int32_t transpose;
+    const void* values_input;
+    void* values_output;
+    int64_t batch;
+} rg_apply_job;
+
+/* Replaces the loop of _regrid/_regrid_from_weights.py:172-182 over
+ * orthogonal elements when the elements carry different weights (the
+ * object array returned by weights() for curvilinear grids that vary along
+ * the orthogonal axes): every job's apply in one launch.  Each job's result
+ * is bit-identical to its own rg_apply_f64 / rg_apply_f32 call. */
+int rg_apply_batched_f64(const rg_apply_job* jobs, int64_t njobs, void* stream);
+int rg_apply_batched_f32(const rg_apply_job* jobs, int64_t njobs, void* stream);
 
 /* Replaces _weights_conservative_2d/_grids.py:356 index_of_point_secant
  * (with its index_of_point_brute fallback, :223), vectorised over n query

@@ -15,9 +15,9 @@ import ctypes
 
 import numpy as np
 
-from ._lib import DeviceArray, Weights, check, lib, rg_tuning, RG_TUNE_TIMINGS
+from ._lib import DeviceArray, Weights, check, lib, rg_pair_desc, rg_tuning, RG_TUNE_TIMINGS
 
-__all__ = ["build_2d", "build_2d_device"]
+__all__ = ["build_2d", "build_2d_device", "build_2d_batched", "pack_pairs"]
 
 
 def build_2d_device(
@@ -68,3 +68,79 @@ def build_2d(
         DeviceArray.from_host(xo), DeviceArray.from_host(yo), xo.shape,
         wi, **tuning,
     )
+
+
+class PackedPairs:
+    """The coordinates (and input weightings) of many grid pairs in one
+    device buffer, with the descriptor array rg_build_conservative_2d_batched
+    takes.  Keep it alive while builds read it."""
+
+    def __init__(self, pairs):
+        arrays, shapes = [], []
+        for item in pairs:
+            gi, go = item[0], item[1]
+            wi = item[2] if len(item) > 2 else None
+            xi = np.ascontiguousarray(gi[0], dtype=np.float64)
+            yi = np.ascontiguousarray(gi[1], dtype=np.float64)
+            xo = np.ascontiguousarray(go[0], dtype=np.float64)
+            yo = np.ascontiguousarray(go[1], dtype=np.float64)
+            if xi.ndim != 2 or xo.ndim != 2 or xi.shape != yi.shape or xo.shape != yo.shape:
+                raise ValueError("2-D grids need two coordinate arrays of one 2-D shape")
+            if wi is not None:
+                wi = np.ascontiguousarray(wi, dtype=np.float64)
+                if wi.shape != (xi.shape[0] - 1, xi.shape[1] - 1):
+                    raise ValueError(
+                        f"weights_input has shape {wi.shape}, expected the input cell "
+                        f"shape {(xi.shape[0] - 1, xi.shape[1] - 1)}"
+                    )
+            arrays.append((xi, yi, xo, yo, wi))
+            shapes.append((xi.shape, xo.shape))
+        sizes = [sum(a.size for a in arr if a is not None) for arr in arrays]
+        flat = np.empty(max(sum(sizes), 1), dtype=np.float64)
+        offsets = []
+        pos = 0
+        for arr in arrays:
+            offs = []
+            for a in arr:
+                if a is None:
+                    offs.append(None)
+                    continue
+                flat[pos : pos + a.size] = a.ravel()
+                offs.append(pos)
+                pos += a.size
+            offsets.append(offs)
+        self.buffer = DeviceArray.from_host(flat)
+        self.n = len(arrays)
+        self.desc = (rg_pair_desc * max(self.n, 1))()
+        base = self.buffer.ptr
+        for k, ((si, so), offs) in enumerate(zip(shapes, offsets)):
+            d = self.desc[k]
+            d.xi, d.yi = base + 8 * offs[0], base + 8 * offs[1]
+            d.nxi, d.nyi = si
+            d.xo, d.yo = base + 8 * offs[2], base + 8 * offs[3]
+            d.nxo, d.nyo = so
+            d.wi = None if offs[4] is None else base + 8 * offs[4]
+
+
+def pack_pairs(pairs) -> PackedPairs:
+    """Copy ``[(grid_input, grid_output[, weights_input]), ...]`` (host
+    arrays, already normalised and perturbed) to the device in one transfer."""
+    return PackedPairs(pairs)
+
+
+def build_2d_batched(pairs, edges_per_lane: int = 0, block: int = 0,
+                     timings: bool = False, stage_per_edge: int = 0,
+                     stream: int = 0) -> list[Weights]:
+    """Build many independent grid pairs together (one launch per phase for
+    the whole batch).  ``pairs`` is a PackedPairs or a sequence accepted by
+    pack_pairs.  Returns one handle per pair, each bit-identical to the
+    pair's own build_2d."""
+    packed = pairs if isinstance(pairs, PackedPairs) else PackedPairs(pairs)
+    if packed.n == 0:
+        return []
+    t = rg_tuning(int(edges_per_lane), int(block), RG_TUNE_TIMINGS if timings else 0,
+                  int(stage_per_edge))
+    hs = (ctypes.c_void_p * packed.n)()
+    check(lib().rg_build_conservative_2d_batched(packed.desc, packed.n, ctypes.byref(t), hs,
+                                                 stream))
+    return [Weights(h) for h in hs]

@@ -49,6 +49,30 @@ class rg_tuning(ctypes.Structure):
     ]
 
 
+class rg_pair_desc(ctypes.Structure):
+    _fields_ = [
+        ("xi", ctypes.c_void_p),
+        ("yi", ctypes.c_void_p),
+        ("nxi", ctypes.c_int64),
+        ("nyi", ctypes.c_int64),
+        ("xo", ctypes.c_void_p),
+        ("yo", ctypes.c_void_p),
+        ("nxo", ctypes.c_int64),
+        ("nyo", ctypes.c_int64),
+        ("wi", ctypes.c_void_p),
+    ]
+
+
+class rg_apply_job(ctypes.Structure):
+    _fields_ = [
+        ("weights", ctypes.c_void_p),
+        ("transpose", ctypes.c_int32),
+        ("values_input", ctypes.c_void_p),
+        ("values_output", ctypes.c_void_p),
+        ("batch", ctypes.c_int64),
+    ]
+
+
 class rg_build_stats(ctypes.Structure):
     _fields_ = [
         ("n_triples", ctypes.c_int64),
@@ -98,6 +122,18 @@ SIGNATURES = {
     "rg_build_conservative_2d": (
         ctypes.c_int,
         [_vp, _vp, _i64, _i64, _vp, _vp, _i64, _i64, _vp, ctypes.POINTER(rg_tuning), ctypes.POINTER(_vp), _vp],
+    ),
+    "rg_build_conservative_2d_batched": (
+        ctypes.c_int,
+        [ctypes.POINTER(rg_pair_desc), _i64, ctypes.POINTER(rg_tuning), ctypes.POINTER(_vp), _vp],
+    ),
+    "rg_apply_batched_f64": (ctypes.c_int, [ctypes.POINTER(rg_apply_job), _i64, _vp]),
+    "rg_apply_batched_f32": (ctypes.c_int, [ctypes.POINTER(rg_apply_job), _i64, _vp]),
+    "rg_tune_build": (
+        ctypes.c_int,
+        [ctypes.POINTER(rg_pair_desc), _i64, _vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int,
+         ctypes.POINTER(rg_tuning), ctypes.POINTER(rg_tuning), ctypes.POINTER(ctypes.c_double),
+         _vp, _vp],
     ),
     "rg_weights_info": (ctypes.c_int, [_vp, _pi64, _pi64, _pi64]),
     "rg_weights_stats": (ctypes.c_int, [_vp, ctypes.POINTER(rg_build_stats)]),

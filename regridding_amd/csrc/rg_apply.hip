@@ -27,7 +27,10 @@
 // the gathered footprint per pass is DB images deep.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstring>
 #include <mutex>
+#include <vector>
 
 #include "rg_internal.hpp"
 
@@ -92,6 +95,7 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_apply(
 
 constexpr int LDS_BYTES = 64 * 1024;  // staged patch per block
 constexpr int STAGE_UNROLL = 8;
+constexpr int64_t PLAN_MIN_ROWS = 65536;
 
 // a / b from a float reciprocal and one correction step (the float quotient
 // is off by at most one).  Exact for 0 <= a <= 65536 and 1 <= b <= 16384,
@@ -287,6 +291,73 @@ __global__ void k_to_f32(const double* __restrict__ a, float* __restrict__ b,
         b[i] = (float)a[i];
 }
 
+// ---- batched apply ---------------------------------------------------------
+//
+// Many applies, each with its own weights, in one launch.  Jobs are cut into
+// blocks of APPLY_BLOCK output rows x DB images; block b finds its job in the
+// jobs' block prefix.  The accumulation per output is the one of k_apply.
+struct JobDesc {
+    const int32_t* rowptr;
+    const int32_t* col;
+    const void* w;
+    const void* x;
+    void* y;
+    int64_t nrows, ncols, batch;
+    int64_t block_base;  // first block of this job
+    int64_t nrt;         // row tiles of the job
+};
+
+constexpr int JOB_DB = 8;
+
+template <class T>
+__global__ __launch_bounds__(APPLY_BLOCK) void k_apply_jobs(
+    const JobDesc* __restrict__ jobs, const int64_t* __restrict__ block_prefix,
+    int njobs)
+{
+    const int64_t b = blockIdx.x;
+    const JobDesc& J = jobs[seg_of(block_prefix, njobs, b)];
+    const int64_t lt = b - J.block_base;
+    const int64_t grp = lt / J.nrt, rt = lt - grp * J.nrt;
+    const int64_t r = rt * APPLY_BLOCK + threadIdx.x;
+    if (r >= J.nrows) return;
+    const int64_t d0 = grp * JOB_DB;
+    const int64_t nd = J.batch - d0 < JOB_DB ? J.batch - d0 : JOB_DB;
+    const int32_t* __restrict__ col = J.col;
+    const T* __restrict__ w = (const T*)J.w;
+    const T* __restrict__ xb = (const T*)J.x + d0 * J.ncols;
+    const int64_t ncols = J.ncols;
+    const int32_t s = J.rowptr[r], e = J.rowptr[r + 1];
+    T acc[JOB_DB];
+#pragma unroll
+    for (int m = 0; m < JOB_DB; m++) acc[m] = T(0);
+    int32_t q = s;
+    for (; q + 4 <= e; q += 4) {
+        int64_t c[4];
+        T wq[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            c[u] = col[q + u];
+            wq[u] = w[q + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int m = 0; m < JOB_DB; m++)
+                if (m < nd) acc[m] = acc[m] + wq[u] * xb[m * ncols + c[u]];
+    }
+    for (; q < e; q++) {
+        const int64_t c = col[q];
+        const T wq = w[q];
+#pragma unroll
+        for (int m = 0; m < JOB_DB; m++)
+            if (m < nd) acc[m] = acc[m] + wq * xb[m * ncols + c];
+    }
+    T* yb = (T*)J.y + d0 * J.nrows + r;
+#pragma unroll
+    for (int m = 0; m < JOB_DB; m++)
+        if (m < nd) yb[m * J.nrows] = acc[m];
+}
+
 namespace {
 
 int g_tile_w = 16;          // tuned on the plain 2049^2 -> 1537^2 pair
@@ -319,7 +390,10 @@ int ensure_plan(rg_weights* W, int dir, const int32_t* rowptr,
                 hipStream_t s, rg_weights::Plan** out)
 {
     *out = nullptr;
-    if (!g_use_plan || G.gx == 1 || W->nnz == 0) return RG_OK;
+    // below PLAN_MIN_ROWS outputs the gathered input stays in L2 anyway and
+    // a plan (two kernels and a host wait) would only add latency
+    if (!g_use_plan || G.gx == 1 || W->nnz == 0 || G.gx * G.gy < PLAN_MIN_ROWS)
+        return RG_OK;
     std::lock_guard<std::mutex> lk(g_plan_mu);
     rg_weights::Plan& P = W->plan[dir];
     if (P.tile_w != G.tw || P.ntiles != G.ntiles) {
@@ -424,6 +498,88 @@ int check_apply(rg_weights* W, const void* vin, void* vout, int64_t batch)
     return RG_OK;
 }
 
+// Pinned staging of a batched apply's job table, one per host thread; the
+// event marks when the previous upload from it has been consumed.
+struct JobUpload {
+    void* buf = nullptr;
+    size_t cap = 0;
+    hipEvent_t done = nullptr;
+    bool recorded = false;
+};
+
+int launch_jobs_common(const rg_apply_job* jobs, int64_t njobs, bool f32,
+                       hipStream_t s)
+{
+    if (njobs < 0) return fail(RG_ERR_ARGUMENT, "negative job count");
+    if (njobs > 0 && !jobs) return fail(RG_ERR_ARGUMENT, "jobs is NULL");
+    std::vector<JobDesc> table;
+    std::vector<int64_t> prefix;
+    table.reserve(njobs);
+    int64_t nblocks = 0;
+    for (int64_t k = 0; k < njobs; k++) {
+        const rg_apply_job& jb = jobs[k];
+        rg_weights* W = jb.weights;
+        RG_TRY(check_apply(W, jb.values_input, jb.values_output, jb.batch));
+        if (f32) RG_TRY(ensure_f32(W, s));
+        JobDesc d;
+        if (jb.transpose) {
+            d.rowptr = W->rowptr_in;
+            d.col = W->col_in;
+            d.w = f32 ? (const void*)W->val_in_f32 : (const void*)W->val_in;
+            d.nrows = W->n_in;
+            d.ncols = W->n_out;
+        } else {
+            d.rowptr = W->rowptr_out;
+            d.col = W->col_out;
+            d.w = f32 ? (const void*)W->val_out_f32 : (const void*)W->val_out;
+            d.nrows = W->n_out;
+            d.ncols = W->n_in;
+        }
+        d.x = jb.values_input;
+        d.y = jb.values_output;
+        d.batch = jb.batch;
+        if (d.nrows == 0 || d.batch == 0) continue;
+        d.nrt = (d.nrows + APPLY_BLOCK - 1) / APPLY_BLOCK;
+        d.block_base = nblocks;
+        prefix.push_back(nblocks);
+        nblocks += d.nrt * ((d.batch + JOB_DB - 1) / JOB_DB);
+        table.push_back(d);
+    }
+    if (table.empty()) return RG_OK;
+    if (nblocks >= INT32_MAX) return fail(RG_ERR_ARGUMENT, "batched apply too large");
+    const size_t tbytes = table.size() * sizeof(JobDesc);
+    const size_t pbytes = prefix.size() * sizeof(int64_t);
+    const size_t poff = (tbytes + 255) / 256 * 256, bytes = poff + pbytes;
+    static thread_local JobUpload up;
+    if (up.recorded) RG_HIP(hipEventSynchronize(up.done));
+    if (bytes > up.cap) {
+        if (up.buf) (void)hipHostFree(up.buf);
+        up.buf = nullptr;
+        up.cap = 0;
+        const size_t sz = std::max<size_t>(bytes, 1 << 16);
+        RG_HIP(hipHostMalloc(&up.buf, sz, hipHostMallocDefault));
+        up.cap = sz;
+    }
+    if (!up.done) RG_HIP(hipEventCreateWithFlags(&up.done, hipEventDisableTiming));
+    std::memcpy(up.buf, table.data(), tbytes);
+    std::memcpy((char*)up.buf + poff, prefix.data(), pbytes);
+    DevBuf<unsigned char> dt;
+    RG_TRY(dt.alloc(bytes, s));
+    RG_HIP(hipMemcpyAsync(dt.p, up.buf, bytes, hipMemcpyHostToDevice, s));
+    RG_HIP(hipEventRecord(up.done, s));
+    up.recorded = true;
+    const JobDesc* d_jobs = (const JobDesc*)dt.p;
+    const int64_t* d_prefix = (const int64_t*)(dt.p + poff);
+    if (f32)
+        k_apply_jobs<float><<<(unsigned)nblocks, APPLY_BLOCK, 0, s>>>(
+            d_jobs, d_prefix, (int)table.size());
+    else
+        k_apply_jobs<double><<<(unsigned)nblocks, APPLY_BLOCK, 0, s>>>(
+            d_jobs, d_prefix, (int)table.size());
+    RG_HIP(hipGetLastError());
+    return RG_OK;  // dt returns to the stream's pool after the kernel
+}
+
 }  // namespace
 
 }  // namespace rg
@@ -461,6 +617,16 @@ int rg_apply_f32(rg_weights_t w, int transpose, const float* values_input,
     return launch_apply<float>(w, 0, w->rowptr_out, w->col_out, w->val_out_f32,
                                values_input, values_output, w->n_out, w->n_in,
                                w->shape_out, w->shape_in[1], batch, s);
+}
+
+int rg_apply_batched_f64(const rg_apply_job* jobs, int64_t njobs, void* stream)
+{
+    return launch_jobs_common(jobs, njobs, false, (hipStream_t)stream);
+}
+
+int rg_apply_batched_f32(const rg_apply_job* jobs, int64_t njobs, void* stream)
+{
+    return launch_jobs_common(jobs, njobs, true, (hipStream_t)stream);
 }
 
 int rg_set_apply_tuning(int tile_w, int images_per_pass, int lds_kib)

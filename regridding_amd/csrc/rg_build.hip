@@ -22,8 +22,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <chrono>
+#include <cstring>
+#include <memory>
 #include <string>
+#include <vector>
 
 #include "rg_internal.hpp"
 #include "rg_sweep.hpp"
@@ -192,18 +194,61 @@ __device__ inline double nan_max(double a, double b)
 }
 
 constexpr int BBOX_BLOCK = 256;
+constexpr int64_t VTILE = 4096;  // vertices per prepare tile
 
-__global__ void k_bbox_partial(const double* __restrict__ x,
-                               const double* __restrict__ y, int64_t n,
-                               double* __restrict__ part)
+// A static grid of a (batched) build and where its derived tables start in
+// the concatenated arrays.
+struct GridDesc {
+    const double* x;
+    const double* y;
+    int64_t nx, ny;
+    int64_t v_base;     // first vertex in xy_all
+    int64_t e_base;     // first boundary edge (bpx, bpy, be, bcell)
+    int64_t blk_base;   // first edge block box
+    int64_t sup_base;   // first super-block box
+    int64_t tile_base;  // first vertex tile
+};
+
+// The cells of one pair of a batched build.
+struct PairCells {
+    const double* xi;
+    const double* yi;
+    int64_t nyi;
+    const double* wi;
+    int64_t cell_gbase, n_in;  // input cells: first (padded) global id, count
+    int64_t out_gbase, n_out;  // output cells
+};
+
+// unit -> segment lookup table (pair of a wavefront, row, cell group ...)
+__global__ void k_segments(const int64_t* __restrict__ prefix, int n,
+                           int64_t nunits, int32_t* __restrict__ out)
 {
-    double v[4] = {x[0], y[0], x[0], y[0]};
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        v[0] = nan_min(v[0], x[i]);
-        v[1] = nan_min(v[1], y[i]);
-        v[2] = nan_max(v[2], x[i]);
-        v[3] = nan_max(v[3], y[i]);
+    const int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (u < nunits) out[u] = seg_of(prefix, n, u);
+}
+
+// One block per tile of VTILE vertices of one grid: interleaves the
+// vertices into (x, y) pairs and reduces the tile's bounding box.
+__global__ __launch_bounds__(BBOX_BLOCK) void k_grid_tiles(
+    const GridDesc* __restrict__ grids, const int64_t* __restrict__ tile_prefix,
+    int ngrids, double2* __restrict__ xy_all, double* __restrict__ part)
+{
+    const int64_t tile = blockIdx.x;
+    const GridDesc& G = grids[seg_of(tile_prefix, ngrids, tile)];
+    const int64_t nv = G.nx * G.ny;
+    const int64_t v0 = (tile - G.tile_base) * VTILE;
+    const int64_t v1 = v0 + VTILE < nv ? v0 + VTILE : nv;
+    double v[4] = {G.x[v0], G.y[v0], G.x[v0], G.y[v0]};
+    for (int64_t i = v0 + threadIdx.x; i < v1; i += BBOX_BLOCK) {
+        const double a = G.x[i], c = G.y[i];
+        double2 q;
+        q.x = a;
+        q.y = c;
+        xy_all[G.v_base + i] = q;
+        v[0] = nan_min(v[0], a);
+        v[1] = nan_min(v[1], c);
+        v[2] = nan_max(v[2], a);
+        v[3] = nan_max(v[3], c);
     }
     __shared__ double sh[4][BBOX_BLOCK];
     for (int q = 0; q < 4; q++) sh[q][threadIdx.x] = v[q];
@@ -218,18 +263,22 @@ __global__ void k_bbox_partial(const double* __restrict__ x,
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        for (int q = 0; q < 4; q++) part[4 * blockIdx.x + q] = sh[q][0];
+        for (int q = 0; q < 4; q++) part[4 * tile + q] = sh[q][0];
 }
 
-__global__ void k_bbox_final(const double* __restrict__ part, int nparts,
-                             double* __restrict__ bbox)
+// One block per grid: the bounding box from its tiles' partial boxes.
+__global__ __launch_bounds__(BBOX_BLOCK) void k_grid_bbox(
+    const int64_t* __restrict__ tile_prefix, const double* __restrict__ part,
+    double* __restrict__ bbox_all)
 {
-    double v[4] = {part[0], part[1], part[2], part[3]};
-    for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
-        v[0] = nan_min(v[0], part[4 * b + 0]);
-        v[1] = nan_min(v[1], part[4 * b + 1]);
-        v[2] = nan_max(v[2], part[4 * b + 2]);
-        v[3] = nan_max(v[3], part[4 * b + 3]);
+    const int g = blockIdx.x;
+    const int64_t t0 = tile_prefix[g], t1 = tile_prefix[g + 1];
+    double v[4] = {part[4 * t0], part[4 * t0 + 1], part[4 * t0 + 2], part[4 * t0 + 3]};
+    for (int64_t t = t0 + threadIdx.x; t < t1; t += BBOX_BLOCK) {
+        v[0] = nan_min(v[0], part[4 * t + 0]);
+        v[1] = nan_min(v[1], part[4 * t + 1]);
+        v[2] = nan_max(v[2], part[4 * t + 2]);
+        v[3] = nan_max(v[3], part[4 * t + 3]);
     }
     __shared__ double sh[4][BBOX_BLOCK];
     for (int q = 0; q < 4; q++) sh[q][threadIdx.x] = v[q];
@@ -244,47 +293,44 @@ __global__ void k_bbox_final(const double* __restrict__ part, int nparts,
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        for (int q = 0; q < 4; q++) bbox[q] = sh[q][0];
+        for (int q = 0; q < 4; q++) bbox_all[4 * g + q] = sh[q][0];
 }
 
-__global__ void k_interleave(const double* __restrict__ x,
-                             const double* __restrict__ y, int64_t n,
-                             double2* __restrict__ xy)
+// Boundary polygon and boundary edge table of every grid (each grid has
+// 2(nx-1)+2(ny-1) of both).
+__global__ void k_grid_boundary(const GridDesc* __restrict__ grids,
+                                const int64_t* __restrict__ e_prefix, int ngrids,
+                                int64_t ne_all, double* __restrict__ bpx,
+                                double* __restrict__ bpy, double* __restrict__ be,
+                                int32_t* __restrict__ bcell)
 {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        double2 v;
-        v.x = x[i];
-        v.y = y[i];
-        xy[i] = v;
-    }
-}
-
-// Boundary polygon and boundary edge table (same length 2(nx-1)+2(ny-1)).
-__global__ void k_boundary(const double* __restrict__ x,
-                           const double* __restrict__ y, int64_t nx, int64_t ny,
-                           double* __restrict__ bpx, double* __restrict__ bpy,
-                           double* __restrict__ be, int32_t* __restrict__ bcell)
-{
-    const int64_t n = 2 * (nx - 1) + 2 * (ny - 1);
     const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    int64_t v = boundary_vertex(nx, ny, e);
-    bpx[e] = x[v];
-    bpy[e] = y[v];
-    boundary_edge(x, y, nx, ny, e, be + 4 * e, bcell + e);
+    if (e >= ne_all) return;
+    const GridDesc& G = grids[seg_of(e_prefix, ngrids, e)];
+    const int64_t le = e - G.e_base;
+    const int64_t v = boundary_vertex(G.nx, G.ny, le);
+    bpx[e] = G.x[v];
+    bpy[e] = G.y[v];
+    boundary_edge(G.x, G.y, G.nx, G.ny, le, be + 4 * e, bcell + e);
 }
 
-__global__ void k_edge_boxes(const double* __restrict__ be, int64_t nbe,
-                             Box* __restrict__ bblk, int64_t nblk)
+__global__ void k_grid_edge_boxes(const GridDesc* __restrict__ grids,
+                                  const int64_t* __restrict__ blk_prefix,
+                                  int ngrids, int64_t nblk_all,
+                                  const double* __restrict__ be_all,
+                                  Box* __restrict__ bblk)
 {
     const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (b >= nblk) return;
+    if (b >= nblk_all) return;
+    const GridDesc& G = grids[seg_of(blk_prefix, ngrids, b)];
+    const int64_t nbe = 2 * (G.nx - 1) + 2 * (G.ny - 1);
+    const int64_t lb = b - G.blk_base;
+    const double* be = be_all + 4 * G.e_base;
     Box r;
     r.x0 = r.y0 = __builtin_huge_val();
     r.x1 = r.y1 = -__builtin_huge_val();
-    int64_t e1 = (b + 1) * BND_BLOCK < nbe ? (b + 1) * BND_BLOCK : nbe;
-    for (int64_t e = b * BND_BLOCK; e < e1; e++) {
+    const int64_t e1 = (lb + 1) * BND_BLOCK < nbe ? (lb + 1) * BND_BLOCK : nbe;
+    for (int64_t e = lb * BND_BLOCK; e < e1; e++) {
         const double* q = be + 4 * e;
         // fmin/fmax ignore NaN; an edge with a NaN end never hits anyway
         r.x0 = fmin(r.x0, fmin(q[0], q[2]));
@@ -295,22 +341,52 @@ __global__ void k_edge_boxes(const double* __restrict__ be, int64_t nbe,
     bblk[b] = r;
 }
 
-__global__ void k_super_boxes(const Box* __restrict__ bblk, int64_t nblk,
-                              Box* __restrict__ bsup, int64_t nsup)
+__global__ void k_grid_super_boxes(const GridDesc* __restrict__ grids,
+                                   const int64_t* __restrict__ sup_prefix,
+                                   int ngrids, int64_t nsup_all,
+                                   const Box* __restrict__ bblk_all,
+                                   Box* __restrict__ bsup)
 {
-    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (s >= nsup) return;
+    const int64_t sidx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (sidx >= nsup_all) return;
+    const GridDesc& G = grids[seg_of(sup_prefix, ngrids, sidx)];
+    const int64_t nbe = 2 * (G.nx - 1) + 2 * (G.ny - 1);
+    const int64_t nblk = (nbe + BND_BLOCK - 1) / BND_BLOCK;
+    const int64_t ls = sidx - G.sup_base;
+    const Box* bblk = bblk_all + G.blk_base;
     Box r;
     r.x0 = r.y0 = __builtin_huge_val();
     r.x1 = r.y1 = -__builtin_huge_val();
-    int64_t b1 = (s + 1) * BND_SUPER < nblk ? (s + 1) * BND_SUPER : nblk;
-    for (int64_t b = s * BND_SUPER; b < b1; b++) {
+    const int64_t b1 = (ls + 1) * BND_SUPER < nblk ? (ls + 1) * BND_SUPER : nblk;
+    for (int64_t b = ls * BND_SUPER; b < b1; b++) {
         r.x0 = fmin(r.x0, bblk[b].x0);
         r.y0 = fmin(r.y0, bblk[b].y0);
         r.x1 = fmax(r.x1, bblk[b].x1);
         r.y1 = fmax(r.y1, bblk[b].y1);
     }
-    bsup[s] = r;
+    bsup[sidx] = r;
+}
+
+// Input cell areas (and the input weighting, 1 where a pair has none) over
+// the padded global cell space; padding cells get area 1 and weight 1 (they
+// receive no triples).
+__global__ void k_pair_cells(const PairCells* __restrict__ pcs,
+                             const int32_t* __restrict__ pair_of_group,
+                             int64_t ncells_all, double* __restrict__ vol,
+                             double* __restrict__ wi_all)
+{
+    const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (c >= ncells_all) return;
+    const PairCells& P = pcs[pair_of_group[c >> 6]];
+    const int64_t lc = c - P.cell_gbase;
+    double a = 1, w = 1;
+    if (lc < P.n_in) {
+        const int64_t ncy = P.nyi - 1;
+        a = cell_area(P.xi, P.yi, P.nyi, lc / ncy, lc % ncy);
+        if (P.wi) w = P.wi[lc];
+    }
+    vol[c] = a;
+    if (wi_all) wi_all[c] = w;
 }
 
 // ---- start states --------------------------------------------------------
@@ -318,17 +394,21 @@ __global__ void k_super_boxes(const Box* __restrict__ bblk, int64_t nblk,
 // Row start of the reference (:293-322), one wavefront per row: bounding
 // box, winding number over the static boundary split across the lanes,
 // secant search on lane 0 with the exhaustive fallback spread over the wave.
-__global__ void k_row_start(Build B, int32_t* __restrict__ start,
+__global__ void k_row_start(const Build* __restrict__ builds,
+                            const int32_t* __restrict__ pair_of_row,
+                            int64_t nrows_all, int32_t* __restrict__ start,
                             int32_t* __restrict__ anomalies)
 {
     const int64_t gw = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
-    if (gw >= B.nrows_total) return;  // uniform per wavefront
-    const int p = pass_of_row(B, gw);
+    if (gw >= nrows_all) return;  // uniform per wavefront
+    const Build& B = builds[__builtin_amdgcn_readfirstlane(pair_of_row[gw])];
+    const int64_t lr = gw - B.row_gbase;
+    const int p = pass_of_row(B, lr);
     const Pass& P = B.pass[p];
     const StaticGrid& s = P.st;
-    const int64_t row = gw - B.row_base[p];
-    const int64_t g = P.chunk_base + row * P.cpr;
+    const int64_t row = lr - B.row_base[p];
+    const int64_t g = B.chunk_gbase + P.chunk_base + row * P.cpr;
     const double px = sweep_x(P, row, 0), py = sweep_y(P, row, 0);
     int32_t state = STATE_OUTSIDE;
     if (s.bbox[0] <= px && px <= s.bbox[2] && s.bbox[1] <= py &&
@@ -356,13 +436,25 @@ __global__ void k_row_start(Build B, int32_t* __restrict__ start,
     if (lane == 0) start[g] = state;
 }
 
-__global__ void k_chunk_start(Build B, int32_t* __restrict__ start)
+// The pair of walk thread t of a batched build (wavefront-uniform: every
+// pair's walk threads are whole wavefronts).
+__device__ inline const Build& build_of_thread(const Build* __restrict__ builds,
+                                               const int32_t* __restrict__ pair_of_wave,
+                                               int64_t t)
+{
+    return builds[__builtin_amdgcn_readfirstlane(pair_of_wave[t >> 6])];
+}
+
+__global__ void k_chunk_start(const Build* __restrict__ builds,
+                              const int32_t* __restrict__ pair_of_wave,
+                              int64_t nthreads_all, int32_t* __restrict__ start)
 {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (t >= B.nthreads) return;
-    const ChunkRef r = chunk_of_thread(B, t);
+    if (t >= nthreads_all) return;
+    const Build& B = build_of_thread(builds, pair_of_wave, t);
+    const ChunkRef r = chunk_of_thread(B, t - B.thread_gbase);
     if (!r.valid || r.c == 0) return;
-    start[r.g] = chunk_start_guess(B.pass[r.p], r.row, r.c * B.K);
+    start[B.chunk_gbase + r.g] = chunk_start_guess(B.pass[r.p], r.row, r.c * B.K);
 }
 
 // ---- walks ---------------------------------------------------------------
@@ -376,38 +468,48 @@ __device__ inline void chunk_range(const Build& B, const Pass& P, int64_t c,
     if (*k1 < *k0) *k1 = *k0;
 }
 
-__global__ void k_walk_count(Build B, const int32_t* __restrict__ start,
+__global__ void k_walk_count(const Build* __restrict__ builds,
+                             const int32_t* __restrict__ pair_of_wave,
+                             int64_t nthreads_all,
+                             const int32_t* __restrict__ start,
                              int32_t* __restrict__ end,
                              int32_t* __restrict__ count,
                              int32_t* __restrict__ hist,
                              Triple* __restrict__ stage, int64_t cap)
 {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (t >= B.nthreads) return;
-    const ChunkRef r = chunk_of_thread(B, t);
+    if (t >= nthreads_all) return;
+    const Build& B = build_of_thread(builds, pair_of_wave, t);
+    const ChunkRef r = chunk_of_thread(B, t - B.thread_gbase);
     if (!r.valid) return;
     const Pass& P = B.pass[r.p];
+    const int64_t g = B.chunk_gbase + r.g;
     int64_t k0, k1;
     chunk_range(B, P, r.c, &k0, &k1);
-    StageSink sink{0, t, cap, stage, hist, r.p, {}};
-    end[r.g] = walk_chunk(B, P, r.row, k0, k1, start[r.g], sink);
+    StageSink sink{0, t, cap, stage, hist + 4 * B.cell_gbase, r.p, {}};
+    end[g] = walk_chunk(B, P, r.row, k0, k1, start[g], sink);
     sink.flush();
-    count[r.g] = sink.n;
+    count[g] = sink.n;
 }
 
 // One thread per row: make every chunk start where its predecessor ended.
-__global__ void k_fixup(Build B, int32_t* __restrict__ start,
+__global__ void k_fixup(const Build* __restrict__ builds,
+                        const int32_t* __restrict__ pair_of_row,
+                        int64_t nrows_all, int32_t* __restrict__ start,
                         int32_t* __restrict__ end, int32_t* __restrict__ count,
-                        int32_t* __restrict__ hist, int32_t* __restrict__ nfix,
+                        int32_t* __restrict__ hist_all, int32_t* __restrict__ nfix,
                         int32_t* __restrict__ failed,
                         Triple* __restrict__ stage, int64_t cap)
 {
-    const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (r >= B.nrows_total) return;
+    const int64_t gr = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (gr >= nrows_all) return;
+    const Build& B = builds[pair_of_row[gr]];
+    const int64_t r = gr - B.row_gbase;
     const int p = pass_of_row(B, r);
     const Pass& P = B.pass[p];
     const int64_t row = r - B.row_base[p];
-    const int64_t g0 = P.chunk_base + row * P.cpr;
+    const int64_t g0 = B.chunk_gbase + P.chunk_base + row * P.cpr;
+    int32_t* hist = hist_all + 4 * B.cell_gbase;
     int32_t fixes = 0;
     for (int64_t c = 1; c < P.cpr; c++) {
         const int64_t g = g0 + c;
@@ -419,7 +521,8 @@ __global__ void k_fixup(Build B, int32_t* __restrict__ start,
         UncountSink undo{hist, p, {}};
         (void)walk_chunk(B, P, row, k0, k1, start[g], undo);
         undo.flush();
-        StageSink redo{0, thread_of_chunk(P, row, c), cap, stage, hist, p, {}};
+        StageSink redo{0, B.thread_gbase + thread_of_chunk(P, row, c), cap, stage,
+                       hist, p, {}};
         end[g] = walk_chunk(B, P, row, k0, k1, truth, redo);
         redo.flush();
         count[g] = redo.n;
@@ -438,27 +541,33 @@ __global__ void k_fixup(Build B, int32_t* __restrict__ start,
 // ordinal (the chunk's first ordinal plus its position in the chunk).
 // Chunks whose triples overflowed the staging walk again, emitting straight
 // into the buckets.
-__global__ void k_scatter(Build B, const int32_t* __restrict__ start,
+__global__ void k_scatter(const Build* __restrict__ builds,
+                          const int32_t* __restrict__ pair_of_wave,
+                          int64_t nthreads_all,
+                          const int32_t* __restrict__ start,
                           const int32_t* __restrict__ count,
                           const Triple* __restrict__ stage, int64_t cap,
                           const int64_t* __restrict__ chunk_off,
-                          uint32_t* __restrict__ cursor,
+                          uint32_t* __restrict__ cursor_all,
                           KV* __restrict__ rec, uint32_t n_rec,
                           int32_t* __restrict__ overflow,
                           int32_t* __restrict__ restaged)
 {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (t >= B.nthreads) return;
-    const ChunkRef r = chunk_of_thread(B, t);
+    if (t >= nthreads_all) return;
+    const Build& B = build_of_thread(builds, pair_of_wave, t);
+    const ChunkRef r = chunk_of_thread(B, t - B.thread_gbase);
     if (!r.valid) return;
-    const int32_t n = count[r.g];
-    const uint32_t ord = (uint32_t)chunk_off[r.g];
+    const int64_t g = B.chunk_gbase + r.g;
+    uint32_t* cursor = cursor_all + 4 * B.cell_gbase;
+    const int32_t n = count[g];
+    const uint32_t ord = (uint32_t)chunk_off[g];
     if (n > cap) {
         const Pass& P = B.pass[r.p];
         int64_t k0, k1;
         chunk_range(B, P, r.c, &k0, &k1);
         BucketSink sink{ord, cursor, rec, n_rec, overflow, r.p};
-        (void)walk_chunk(B, P, r.row, k0, k1, start[r.g], sink);
+        (void)walk_chunk(B, P, r.row, k0, k1, start[g], sink);
         atomicAdd(restaged, 1);
         return;
     }
@@ -566,12 +675,58 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
         }
         if (lane == 0) group_mode[grp] = packed ? 0 : 1;
     } else {
-        // oversized group: each lane finishes its bucket in global memory
-        if (i < i1) {
-            KV* r = rec + base + s;
-            for (int64_t q = 0; q < n; q++) r[q].val = weight_of(r[q].val, vol, wi, i);
-            sort_records(r, n);
-            nnz_row[i] = (int32_t)bucket_reduce_long(r, n);
+        // Oversized group: the cells go through the LDS in runs of
+        // consecutive cells that fit it, each cell reduced in place at its
+        // bucket start (the group is not packed).  A single cell larger
+        // than the LDS share is finished in global memory by its lane.
+        uint64_t* K = sk[wid];
+        double* V = sv[wid];
+        int64_t c = i0;
+        while (c < i1) {
+            const int64_t sb = bucket_off[4 * c] - base;  // wave-uniform
+            const bool in = i >= c && i < i1 && s + n - sb <= RED_CAP;
+            const uint64_t mask = __ballot(in);
+            if (mask == 0) {
+                if (i == c) {
+                    KV* r = rec + base + s;
+                    for (int64_t q = 0; q < n; q++)
+                        r[q].val = weight_of(r[q].val, vol, wi, i);
+                    sort_records(r, n);
+                    nnz_row[i] = (int32_t)bucket_reduce_long(r, n);
+                }
+                c++;
+                continue;
+            }
+            const int64_t d = c + __popcll(mask);  // cells [c, d) fit
+            const int64_t tot = bucket_off[4 * d] - base - sb;
+            for (int64_t q = lane; q < tot; q += 64) {
+                const KV r = rec[base + sb + q];
+                K[q] = r.key;
+                V[q] = r.val;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (in) {
+                uint64_t* Kc = K + (s - sb);
+                double* Vc = V + (s - sb);
+                for (int64_t q = 0; q < n; q++) Vc[q] = weight_of(Vc[q], vol, wi, i);
+                bool lr = false;
+                int64_t m = bucket_reduce(Kc, Vc, n, &lr);
+                if (lr) m = n;  // left sorted, uncompacted, for k_bucket_long
+                for (int64_t t = 0; t < m; t++) {
+                    KV r;
+                    r.key = Kc[t];
+                    r.val = Vc[t];
+                    rec[base + s + t] = r;
+                }
+                nnz_row[i] = (int32_t)m;
+                if (lr) long_list[atomicAdd(n_long, 1)] = (int32_t)i;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            c = d;
         }
         if (lane == 0) group_mode[grp] = 1;
     }
@@ -598,27 +753,35 @@ __global__ void k_bucket_long(const int64_t* __restrict__ bucket_off,
 
 // CSR by input from the reduced groups: a packed group is one contiguous
 // copy; a fallback group copies bucket by bucket.  Also counts the entries
-// of every output cell for the transpose.
-__global__ void k_csr_groups(const int64_t* __restrict__ bucket_off,
+// of every output cell for the transpose, and writes each pair's int32 row
+// pointers (relative to the pair's first entry; pair k's n_in + 1 pointers
+// start at rowptr_all[cell_gbase + k]).
+__global__ void k_csr_groups(const PairCells* __restrict__ pcs,
+                             const int32_t* __restrict__ pair_of_group,
+                             int64_t n_groups,
+                             const int64_t* __restrict__ bucket_off,
                              const int64_t* __restrict__ row_off,
                              const int32_t* __restrict__ group_mode,
-                             const KV* __restrict__ rec, int64_t n_in,
-                             int32_t* __restrict__ rowptr,
+                             const KV* __restrict__ rec,
+                             int32_t* __restrict__ rowptr_all,
                              int32_t* __restrict__ col,
                              double* __restrict__ out_val,
                              int32_t* __restrict__ hist_out)
 {
     const int64_t grp = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
-    const int64_t i0 = grp * 64;
-    if (i0 > n_in) return;
-    if (i0 == n_in) {  // the closing row pointer
-        if (lane == 0) rowptr[n_in] = (int32_t)row_off[n_in];
-        return;
-    }
-    const int64_t i1 = i0 + 64 < n_in ? i0 + 64 : n_in;
-    if (i0 + lane < i1) rowptr[i0 + lane] = (int32_t)row_off[i0 + lane];
-    if (i1 == n_in && lane == 0) rowptr[n_in] = (int32_t)row_off[n_in];
+    if (grp >= n_groups) return;  // uniform per wavefront
+    const int k = __builtin_amdgcn_readfirstlane(pair_of_group[grp]);
+    const PairCells& P = pcs[k];
+    const int64_t i0 = grp * 64, iend = P.cell_gbase + P.n_in;
+    const int64_t i1 = i0 + 64 < iend ? i0 + 64 : iend;
+    const int64_t ebase = row_off[P.cell_gbase];
+    int32_t* rowptr = rowptr_all + P.cell_gbase + k;
+    const int64_t i = i0 + lane;
+    if (i < i1) rowptr[i - P.cell_gbase] = (int32_t)(row_off[i] - ebase);
+    if (i == iend - 1) rowptr[P.n_in] = (int32_t)(row_off[iend] - ebase);
+    if (i0 >= i1) return;
+    int32_t* hist = hist_out + P.out_gbase;
     const int64_t d0 = row_off[i0];
     if (group_mode[grp] == 0) {
         const int64_t b0 = bucket_off[4 * i0], cnt = row_off[i1] - d0;
@@ -627,40 +790,43 @@ __global__ void k_csr_groups(const int64_t* __restrict__ bucket_off,
             const int32_t o = (int32_t)r.key;
             col[d0 + q] = o;
             out_val[d0 + q] = r.val;
-            atomicAdd(&hist_out[o], 1);
+            atomicAdd(&hist[o], 1);
         }
-    } else if (i0 + lane < i1) {
-        const int64_t i = i0 + lane;
+    } else if (i < i1) {
         const int64_t sb = bucket_off[4 * i], d = row_off[i], m = row_off[i + 1] - d;
         for (int64_t q = 0; q < m; q++) {
             const KV r = rec[sb + q];
             const int32_t o = (int32_t)r.key;
             col[d + q] = o;
             out_val[d + q] = r.val;
-            atomicAdd(&hist_out[o], 1);
+            atomicAdd(&hist[o], 1);
         }
     }
 }
 
-// Transpose scatter: entry q of row i of a CSR goes to row col[q] of the
+// Transpose scatter: entry q of input row i goes to output row col[q] of the
 // transposed CSR at an atomically claimed slot; k_rows_sort then orders each
 // transposed row by column, which makes the result deterministic.
-__global__ void k_transpose_scatter(const int32_t* __restrict__ rowptr,
+__global__ void k_transpose_scatter(const PairCells* __restrict__ pcs,
+                                    const int32_t* __restrict__ pair_of_group,
+                                    int64_t nrows_all,
+                                    const int64_t* __restrict__ row_off,
                                     const int32_t* __restrict__ col,
                                     const double* __restrict__ val,
-                                    int64_t nrows,
                                     const int64_t* __restrict__ t_off,
                                     int32_t* __restrict__ cursor,
                                     int32_t* __restrict__ t_col,
                                     double* __restrict__ t_val)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= nrows) return;
-    for (int32_t q = rowptr[i]; q < rowptr[i + 1]; q++) {
-        int32_t o = col[q];
-        int64_t pos = t_off[o] + atomicAdd(&cursor[o], 1);
+    if (i >= nrows_all) return;
+    const PairCells& P = pcs[__builtin_amdgcn_readfirstlane(pair_of_group[i >> 6])];
+    const int32_t li = (int32_t)(i - P.cell_gbase);
+    for (int64_t q = row_off[i]; q < row_off[i + 1]; q++) {
+        const int64_t o = P.out_gbase + col[q];
+        const int64_t pos = t_off[o] + atomicAdd(&cursor[o], 1);
         if (pos < t_off[o + 1]) {
-            t_col[pos] = (int32_t)i;
+            t_col[pos] = li;
             t_val[pos] = val[q];
         }
     }
@@ -670,23 +836,30 @@ __global__ void k_transpose_scatter(const int32_t* __restrict__ rowptr,
 // in atomic order).  One wavefront per 64 consecutive rows, whose entries
 // are one contiguous range: loaded into LDS with coalesced loads, sorted
 // per lane in LDS, stored back coalesced.  Oversized groups sort in global
-// memory.  Also writes the int32 row pointers.
+// memory.  Also writes each pair's int32 row pointers (pair k's n_out + 1
+// pointers start at rowptr_all[out_gbase + k]).
 constexpr int SORT_WAVES = 4;
 constexpr int SORT_CAP = 1536;  // entries per wavefront (18 KiB)
 
 __global__ __launch_bounds__(SORT_WAVES * 64) void k_rows_sort(
-    const int64_t* __restrict__ off, int64_t nrows, int32_t* __restrict__ col,
-    double* __restrict__ val, int32_t* __restrict__ rowptr)
+    const PairCells* __restrict__ pcs, const int32_t* __restrict__ pair_of_group,
+    const int64_t* __restrict__ off, int64_t nrows_all, int32_t* __restrict__ col,
+    double* __restrict__ val, int32_t* __restrict__ rowptr_all)
 {
     __shared__ int32_t sc[SORT_WAVES][SORT_CAP];
     __shared__ double sv[SORT_WAVES][SORT_CAP];
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t r0 = ((int64_t)blockIdx.x * SORT_WAVES + wid) * 64;
-    if (r0 > nrows) return;  // uniform per wavefront
+    if (r0 >= nrows_all) return;  // uniform per wavefront
+    const int k = __builtin_amdgcn_readfirstlane(pair_of_group[r0 >> 6]);
+    const PairCells& P = pcs[k];
+    const int64_t rend = P.out_gbase + P.n_out;
+    const int64_t obase = off[P.out_gbase];
+    int32_t* rowptr = rowptr_all + P.out_gbase + k;
     const int64_t r = r0 + lane;
-    if (r <= nrows) rowptr[r] = (int32_t)off[r];
-    if (r0 == nrows) return;
-    const int64_t r1 = r0 + 64 < nrows ? r0 + 64 : nrows;
+    if (r < rend) rowptr[r - P.out_gbase] = (int32_t)(off[r] - obase);
+    if (r == rend - 1) rowptr[P.n_out] = (int32_t)(off[rend] - obase);
+    const int64_t r1 = r0 + 64 < nrows_all ? r0 + 64 : nrows_all;
     const int64_t base = off[r0], total = off[r1] - base;
     int64_t s = 0, n = 0;
     if (r < r1) {
@@ -716,6 +889,20 @@ __global__ __launch_bounds__(SORT_WAVES * 64) void k_rows_sort(
     }
 }
 
+// Per pair: its first entry (row_off at its first cell) and its first
+// triple (chunk_off at its first chunk); slot npairs holds the totals.
+__global__ void k_pair_bases(const PairCells* __restrict__ pcs,
+                             const Build* __restrict__ builds, int64_t npairs,
+                             const int64_t* __restrict__ row_off, int64_t ncells_all,
+                             const int64_t* __restrict__ chunk_off,
+                             int64_t nchunks_all, int64_t* __restrict__ out)
+{
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k > npairs) return;
+    out[2 * k] = row_off[k < npairs ? pcs[k].cell_gbase : ncells_all];
+    out[2 * k + 1] = chunk_off[k < npairs ? builds[k].chunk_gbase : nchunks_all];
+}
+
 __global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
@@ -727,13 +914,6 @@ __global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n)
 
 namespace {
 
-struct StaticTables {
-    DevBuf<double2> xy;
-    DevBuf<double> bbox, part, bpx, bpy, be;
-    DevBuf<int32_t> bcell;
-    DevBuf<Box> bblk, bsup;
-};
-
 int fill_zero(int32_t* p, int64_t n, hipStream_t s)
 {
     if (n == 0) return RG_OK;
@@ -743,53 +923,12 @@ int fill_zero(int32_t* p, int64_t n, hipStream_t s)
     return RG_OK;
 }
 
-int prepare_static(const double* x, const double* y, int64_t nx, int64_t ny,
-                   StaticTables& T, StaticGrid& G, hipStream_t s)
+int launch_segments(const int64_t* prefix, int n, int64_t nunits, int32_t* out,
+                    hipStream_t s)
 {
-    const int64_t nv = nx * ny;
-    const int64_t nbe = 2 * (nx - 1) + 2 * (ny - 1);
-    const int64_t nblk = (nbe + BND_BLOCK - 1) / BND_BLOCK;
-    const int64_t nsup = (nblk + BND_SUPER - 1) / BND_SUPER;
-    int nparts = (int)std::min<int64_t>(grid_for(nv, BBOX_BLOCK), 1024);
-    RG_TRY(T.xy.alloc(nv, s));
-    RG_TRY(T.bbox.alloc(4, s));
-    RG_TRY(T.part.alloc(4 * (size_t)nparts, s));
-    RG_TRY(T.bpx.alloc(nbe, s));
-    RG_TRY(T.bpy.alloc(nbe, s));
-    RG_TRY(T.be.alloc(4 * nbe, s));
-    RG_TRY(T.bcell.alloc(nbe, s));
-    RG_TRY(T.bblk.alloc(nblk, s));
-    RG_TRY(T.bsup.alloc(nsup, s));
-    k_interleave<<<std::min<unsigned>(grid_for(nv, 256), 16384), 256, 0, s>>>(
-        x, y, nv, T.xy.p);
-    k_bbox_partial<<<nparts, BBOX_BLOCK, 0, s>>>(x, y, nv, T.part.p);
+    if (nunits == 0) return RG_OK;
+    k_segments<<<grid_for(nunits, 256), 256, 0, s>>>(prefix, n, nunits, out);
     RG_HIP(hipGetLastError());
-    k_bbox_final<<<1, BBOX_BLOCK, 0, s>>>(T.part.p, nparts, T.bbox.p);
-    RG_HIP(hipGetLastError());
-    k_boundary<<<grid_for(nbe, 256), 256, 0, s>>>(x, y, nx, ny, T.bpx.p,
-                                                   T.bpy.p, T.be.p, T.bcell.p);
-    RG_HIP(hipGetLastError());
-    k_edge_boxes<<<grid_for(nblk, 256), 256, 0, s>>>(T.be.p, nbe, T.bblk.p, nblk);
-    RG_HIP(hipGetLastError());
-    k_super_boxes<<<grid_for(nsup, 256), 256, 0, s>>>(T.bblk.p, nblk, T.bsup.p,
-                                                       nsup);
-    RG_HIP(hipGetLastError());
-    G.x = x;
-    G.y = y;
-    G.xy = T.xy.p;
-    G.nx = nx;
-    G.ny = ny;
-    G.bbox = T.bbox.p;
-    G.bpx = T.bpx.p;
-    G.bpy = T.bpy.p;
-    G.nbp = nbe;
-    G.be = T.be.p;
-    G.bcell = T.bcell.p;
-    G.nbe = nbe;
-    G.bblk = T.bblk.p;
-    G.nblk = nblk;
-    G.bsup = T.bsup.p;
-    G.nsup = nsup;
     return RG_OK;
 }
 
@@ -822,25 +961,72 @@ struct PhaseTimer {
     }
 };
 
+// Pinned host staging of the build's descriptor tables: one grow-only
+// buffer per host thread.  The upload is asynchronous; every build waits on
+// its stream (its first scalar read) before it returns, and an early error
+// return waits here, so the next build of the thread may reuse the buffer.
+struct PinnedUpload {
+    hipStream_t s = nullptr;
+    bool pending = false;
+    ~PinnedUpload()
+    {
+        if (pending) (void)hipStreamSynchronize(s);
+    }
+    static int get(void** p, size_t bytes)
+    {
+        static thread_local void* buf = nullptr;
+        static thread_local size_t cap = 0;
+        if (bytes > cap) {
+            if (buf) (void)hipHostFree(buf);
+            buf = nullptr;
+            cap = 0;
+            const size_t sz = std::max<size_t>(bytes, 1 << 16);
+            RG_HIP(hipHostMalloc(&buf, sz, hipHostMallocDefault));
+            cap = sz;
+        }
+        *p = buf;
+        return RG_OK;
+    }
+};
+
+int64_t round64(int64_t n) { return (n + 63) / 64 * 64; }
+
+template <class T>
+size_t append_table(std::vector<unsigned char>& buf, const T* src, size_t n)
+{
+    size_t off = (buf.size() + 255) / 256 * 256;
+    buf.resize(off + n * sizeof(T));
+    if (n) std::memcpy(buf.data() + off, src, n * sizeof(T));
+    return off;
+}
+
 }  // namespace
 
-int build_conservative_2d(const double* xi, const double* yi, int64_t nxi,
-                          int64_t nyi, const double* xo, const double* yo,
-                          int64_t nxo, int64_t nyo, const double* wi,
-                          const rg_tuning* tuning, rg_weights** out,
-                          hipStream_t s)
+// The build of npairs independent grid pairs: every phase is one launch over
+// the concatenation of all pairs' chunks, rows and (padded) cells.  A single
+// pair is a batch of one.
+int build_batched(const rg_pair_desc* pairs, int64_t npairs,
+                  const rg_tuning* tuning, rg_weights** out, hipStream_t s)
 {
-    *out = nullptr;
+    if (npairs <= 0) return fail(RG_ERR_ARGUMENT, "npairs must be positive");
+    if (!pairs || !out) return fail(RG_ERR_ARGUMENT, "pairs or out is NULL");
+    for (int64_t k = 0; k < npairs; k++) out[k] = nullptr;
     if (injected_failure("build"))
         return fail(RG_ERR_INJECTED, "injected build failure");
-    if (!xi || !yi || !xo || !yo)
-        return fail(RG_ERR_ARGUMENT, "coordinate pointer is NULL");
-    if (nxi < 2 || nyi < 2 || nxo < 2 || nyo < 2)
-        return fail(RG_ERR_ARGUMENT,
-                    "each grid needs at least 2 vertices along both axes");
-    const int64_t n_in = (nxi - 1) * (nyi - 1), n_out = (nxo - 1) * (nyo - 1);
-    if (n_in >= INT32_MAX || n_out >= INT32_MAX)
-        return fail(RG_ERR_ARGUMENT, "more than 2^31 - 1 cells in one grid");
+    if (npairs >= (int64_t)1 << 24)
+        return fail(RG_ERR_ARGUMENT, "too many pairs in one batch");
+    for (int64_t k = 0; k < npairs; k++) {
+        const rg_pair_desc& d = pairs[k];
+        const std::string at = npairs > 1 ? " (pair " + std::to_string(k) + ")" : "";
+        if (!d.xi || !d.yi || !d.xo || !d.yo)
+            return fail(RG_ERR_ARGUMENT, "coordinate pointer is NULL" + at);
+        if (d.nxi < 2 || d.nyi < 2 || d.nxo < 2 || d.nyo < 2)
+            return fail(RG_ERR_ARGUMENT,
+                        "each grid needs at least 2 vertices along both axes" + at);
+        if ((d.nxi - 1) * (d.nyi - 1) >= INT32_MAX ||
+            (d.nxo - 1) * (d.nyo - 1) >= INT32_MAX)
+            return fail(RG_ERR_ARGUMENT, "more than 2^31 - 1 cells in one grid" + at);
+    }
     int64_t K = 16, stage_per_edge = 8;
     int block = 128;
     bool timings = false;
@@ -853,99 +1039,257 @@ int build_conservative_2d(const double* xi, const double* yi, int64_t nxi,
     if (block % 64 != 0 || block > 1024)
         return fail(RG_ERR_ARGUMENT, "block must be a multiple of 64, <= 1024");
 
-    auto t_host0 = std::chrono::steady_clock::now();
     PhaseTimer tm(timings, s);
     tm.mark();  // 0
 
-    // ---- prepare
-    DevBuf<double> vol;
-    RG_TRY(vol.alloc(n_in, s));
-    RG_TRY(launch_area(xi, yi, nxi, nyi, vol.p, s));
-    StaticTables tab_in, tab_out;
-    StaticGrid st_in, st_out;
-    RG_TRY(prepare_static(xi, yi, nxi, nyi, tab_in, st_in, s));
-    RG_TRY(prepare_static(xo, yo, nxo, nyo, tab_out, st_out, s));
-
-    Build B;
-    B.K = K;
-    int64_t chunk = 0, rows = 0, threads = 0;
-    for (int p = 0; p < 4; p++) {
-        Pass& P = B.pass[p];
-        const int sweep_input = p >= 2;
-        const int axis = p & 1;
-        const double* xs = sweep_input ? xi : xo;
-        const double* ys = sweep_input ? yi : yo;
-        const int64_t nxs = sweep_input ? nxi : nxo;
-        const int64_t nys = sweep_input ? nyi : nyo;
-        P.xs = xs;
-        P.ys = ys;
-        if (axis == 0) {  // aligned view = transpose
-            P.sr = 1; P.sk = nys; P.nrows = nys; P.nk = nxs;
-        } else {
-            P.sr = nys; P.sk = 1; P.nrows = nxs; P.nk = nys;
-        }
-        P.ncy_sweep = nys - 1;
-        P.axis = axis;
-        P.sweep_input = sweep_input;
-        P.st = sweep_input ? st_out : st_in;
-        P.cpr = std::max<int64_t>(1, (P.nk - 1 + K - 1) / K);
-        P.chunk_base = chunk;
-        P.thread_base = threads;
-        B.row_base[p] = rows;
-        chunk += P.nrows * P.cpr;
-        rows += P.nrows;
-        threads += threads_of_pass(P.nrows, P.cpr);
+    // ---- host tables: grids (2k input, 2k+1 output), cells, passes
+    const int ng = (int)(2 * npairs);
+    std::vector<GridDesc> grids(ng);
+    std::vector<int64_t> tile_pf(ng + 1), e_pf(ng + 1), blk_pf(ng + 1), sup_pf(ng + 1);
+    int64_t v_tot = 0, e_tot = 0, blk_tot = 0, sup_tot = 0, tile_tot = 0;
+    for (int g = 0; g < ng; g++) {
+        const rg_pair_desc& d = pairs[g / 2];
+        GridDesc& G = grids[g];
+        G.x = g % 2 ? d.xo : d.xi;
+        G.y = g % 2 ? d.yo : d.yi;
+        G.nx = g % 2 ? d.nxo : d.nxi;
+        G.ny = g % 2 ? d.nyo : d.nyi;
+        const int64_t nv = G.nx * G.ny, nbe = 2 * (G.nx - 1) + 2 * (G.ny - 1);
+        const int64_t nblk = (nbe + BND_BLOCK - 1) / BND_BLOCK;
+        const int64_t nsup = (nblk + BND_SUPER - 1) / BND_SUPER;
+        G.v_base = v_tot;
+        G.e_base = e_pf[g] = e_tot;
+        G.blk_base = blk_pf[g] = blk_tot;
+        G.sup_base = sup_pf[g] = sup_tot;
+        G.tile_base = tile_pf[g] = tile_tot;
+        v_tot += nv;
+        e_tot += nbe;
+        blk_tot += nblk;
+        sup_tot += nsup;
+        tile_tot += (nv + VTILE - 1) / VTILE;
     }
-    B.nchunks = chunk;
-    B.nrows_total = rows;
-    B.nthreads = threads;
-    if (B.nthreads >= INT32_MAX)
-        return fail(RG_ERR_ARGUMENT, "too many sweep chunks");
-    if (B.nthreads % 64 != 0)  // staging regions are whole wavefronts
-        return fail(RG_ERR_ARGUMENT, "walk thread count is not wavefront-aligned");
+    tile_pf[ng] = tile_tot;
+    e_pf[ng] = e_tot;
+    blk_pf[ng] = blk_tot;
+    sup_pf[ng] = sup_tot;
+
+    DevBuf<double2> xy_all;
+    DevBuf<double> part, bbox_all, bpx, bpy, be;
+    DevBuf<int32_t> bcell;
+    DevBuf<Box> bblk, bsup;
+    RG_TRY(xy_all.alloc(v_tot, s));
+    RG_TRY(part.alloc(4 * tile_tot, s));
+    RG_TRY(bbox_all.alloc(4 * ng, s));
+    RG_TRY(bpx.alloc(e_tot, s));
+    RG_TRY(bpy.alloc(e_tot, s));
+    RG_TRY(be.alloc(4 * e_tot, s));
+    RG_TRY(bcell.alloc(e_tot, s));
+    RG_TRY(bblk.alloc(blk_tot, s));
+    RG_TRY(bsup.alloc(sup_tot, s));
+    std::vector<StaticGrid> sg(ng);
+    for (int g = 0; g < ng; g++) {
+        const GridDesc& G = grids[g];
+        StaticGrid& S = sg[g];
+        S.x = G.x;
+        S.y = G.y;
+        S.xy = xy_all.p + G.v_base;
+        S.nx = G.nx;
+        S.ny = G.ny;
+        S.bbox = bbox_all.p + 4 * g;
+        S.bpx = bpx.p + G.e_base;
+        S.bpy = bpy.p + G.e_base;
+        S.nbp = e_pf[g + 1] - e_pf[g];
+        S.be = be.p + 4 * G.e_base;
+        S.bcell = bcell.p + G.e_base;
+        S.nbe = S.nbp;
+        S.bblk = bblk.p + G.blk_base;
+        S.nblk = blk_pf[g + 1] - blk_pf[g];
+        S.bsup = bsup.p + G.sup_base;
+        S.nsup = sup_pf[g + 1] - sup_pf[g];
+    }
+
+    std::vector<PairCells> pcs(npairs);
+    std::vector<Build> builds(npairs);
+    std::vector<int64_t> wave_pf(npairs + 1), row_pf(npairs + 1),
+        ing_pf(npairs + 1), outg_pf(npairs + 1);
+    int64_t cell_tot = 0, out_tot = 0, thr_tot = 0, chunk_tot = 0, row_tot = 0;
+    bool any_wi = false;
+    for (int64_t k = 0; k < npairs; k++) {
+        const rg_pair_desc& d = pairs[k];
+        PairCells& P = pcs[k];
+        P.xi = d.xi;
+        P.yi = d.yi;
+        P.nyi = d.nyi;
+        P.wi = d.wi;
+        any_wi = any_wi || d.wi;
+        P.n_in = (d.nxi - 1) * (d.nyi - 1);
+        P.n_out = (d.nxo - 1) * (d.nyo - 1);
+        P.cell_gbase = cell_tot;
+        P.out_gbase = out_tot;
+        Build& B = builds[k];
+        B.K = K;
+        int64_t chunk = 0, rows = 0, threads = 0;
+        for (int p = 0; p < 4; p++) {
+            Pass& Q = B.pass[p];
+            const int sweep_input = p >= 2;
+            const int axis = p & 1;
+            const int64_t nxs = sweep_input ? d.nxi : d.nxo;
+            const int64_t nys = sweep_input ? d.nyi : d.nyo;
+            Q.xs = sweep_input ? d.xi : d.xo;
+            Q.ys = sweep_input ? d.yi : d.yo;
+            if (axis == 0) {  // aligned view = transpose
+                Q.sr = 1; Q.sk = nys; Q.nrows = nys; Q.nk = nxs;
+            } else {
+                Q.sr = nys; Q.sk = 1; Q.nrows = nxs; Q.nk = nys;
+            }
+            Q.ncy_sweep = nys - 1;
+            Q.axis = axis;
+            Q.sweep_input = sweep_input;
+            Q.st = sweep_input ? sg[2 * k + 1] : sg[2 * k];
+            Q.cpr = std::max<int64_t>(1, (Q.nk - 1 + K - 1) / K);
+            Q.chunk_base = chunk;
+            Q.thread_base = threads;
+            B.row_base[p] = rows;
+            chunk += Q.nrows * Q.cpr;
+            rows += Q.nrows;
+            threads += threads_of_pass(Q.nrows, Q.cpr);
+        }
+        B.nchunks = chunk;
+        B.nrows_total = rows;
+        B.nthreads = threads;
+        B.thread_gbase = thr_tot;
+        B.chunk_gbase = chunk_tot;
+        B.row_gbase = row_tot;
+        B.cell_gbase = cell_tot;
+        B.out_gbase = out_tot;
+        wave_pf[k] = thr_tot / 64;
+        row_pf[k] = row_tot;
+        ing_pf[k] = cell_tot / 64;
+        outg_pf[k] = out_tot / 64;
+        thr_tot += threads;
+        chunk_tot += chunk;
+        row_tot += rows;
+        cell_tot += round64(P.n_in);
+        out_tot += round64(P.n_out);
+    }
+    wave_pf[npairs] = thr_tot / 64;
+    row_pf[npairs] = row_tot;
+    ing_pf[npairs] = cell_tot / 64;
+    outg_pf[npairs] = out_tot / 64;
+    if (thr_tot >= INT32_MAX || thr_tot % 64 != 0)
+        return fail(RG_ERR_ARGUMENT, "too many sweep chunks in one build");
+
+    // ---- upload the tables (one copy)
+    std::vector<unsigned char> hb;
+    const size_t o_builds = append_table(hb, builds.data(), builds.size());
+    const size_t o_grids = append_table(hb, grids.data(), grids.size());
+    const size_t o_pcs = append_table(hb, pcs.data(), pcs.size());
+    const size_t o_tile = append_table(hb, tile_pf.data(), tile_pf.size());
+    const size_t o_e = append_table(hb, e_pf.data(), e_pf.size());
+    const size_t o_blk = append_table(hb, blk_pf.data(), blk_pf.size());
+    const size_t o_sup = append_table(hb, sup_pf.data(), sup_pf.size());
+    const size_t o_wave = append_table(hb, wave_pf.data(), wave_pf.size());
+    const size_t o_row = append_table(hb, row_pf.data(), row_pf.size());
+    const size_t o_ing = append_table(hb, ing_pf.data(), ing_pf.size());
+    const size_t o_outg = append_table(hb, outg_pf.data(), outg_pf.size());
+    DevBuf<unsigned char> meta;
+    RG_TRY(meta.alloc(hb.size(), s));
+    PinnedUpload up;
+    up.s = s;
+    void* pinned = nullptr;
+    RG_TRY(PinnedUpload::get(&pinned, hb.size()));
+    std::memcpy(pinned, hb.data(), hb.size());
+    up.pending = true;
+    RG_HIP(hipMemcpyAsync(meta.p, pinned, hb.size(), hipMemcpyHostToDevice, s));
+    const Build* d_builds = (const Build*)(meta.p + o_builds);
+    const GridDesc* d_grids = (const GridDesc*)(meta.p + o_grids);
+    const PairCells* d_pcs = (const PairCells*)(meta.p + o_pcs);
+    const int64_t* d_tile = (const int64_t*)(meta.p + o_tile);
+    const int64_t* d_e = (const int64_t*)(meta.p + o_e);
+    const int64_t* d_blk = (const int64_t*)(meta.p + o_blk);
+    const int64_t* d_sup = (const int64_t*)(meta.p + o_sup);
+
+    // ---- prepare
+    DevBuf<int32_t> pair_of_wave, pair_of_row, pair_of_ing, pair_of_outg;
+    RG_TRY(pair_of_wave.alloc(thr_tot / 64, s));
+    RG_TRY(pair_of_row.alloc(row_tot, s));
+    RG_TRY(pair_of_ing.alloc(cell_tot / 64, s));
+    RG_TRY(pair_of_outg.alloc(out_tot / 64, s));
+    const int np32 = (int)npairs;
+    RG_TRY(launch_segments((const int64_t*)(meta.p + o_wave), np32, thr_tot / 64,
+                           pair_of_wave.p, s));
+    RG_TRY(launch_segments((const int64_t*)(meta.p + o_row), np32, row_tot,
+                           pair_of_row.p, s));
+    RG_TRY(launch_segments((const int64_t*)(meta.p + o_ing), np32, cell_tot / 64,
+                           pair_of_ing.p, s));
+    RG_TRY(launch_segments((const int64_t*)(meta.p + o_outg), np32, out_tot / 64,
+                           pair_of_outg.p, s));
+    k_grid_tiles<<<(unsigned)tile_tot, BBOX_BLOCK, 0, s>>>(d_grids, d_tile, ng,
+                                                            xy_all.p, part.p);
+    RG_HIP(hipGetLastError());
+    k_grid_bbox<<<(unsigned)ng, BBOX_BLOCK, 0, s>>>(d_tile, part.p, bbox_all.p);
+    RG_HIP(hipGetLastError());
+    k_grid_boundary<<<grid_for(e_tot, 256), 256, 0, s>>>(d_grids, d_e, ng, e_tot,
+                                                         bpx.p, bpy.p, be.p, bcell.p);
+    RG_HIP(hipGetLastError());
+    k_grid_edge_boxes<<<grid_for(blk_tot, 256), 256, 0, s>>>(d_grids, d_blk, ng,
+                                                             blk_tot, be.p, bblk.p);
+    RG_HIP(hipGetLastError());
+    k_grid_super_boxes<<<grid_for(sup_tot, 256), 256, 0, s>>>(d_grids, d_sup, ng,
+                                                              sup_tot, bblk.p, bsup.p);
+    RG_HIP(hipGetLastError());
+    DevBuf<double> vol, wi_all;
+    RG_TRY(vol.alloc(cell_tot, s));
+    if (any_wi) RG_TRY(wi_all.alloc(cell_tot, s));
+    k_pair_cells<<<grid_for(cell_tot, 256), 256, 0, s>>>(d_pcs, pair_of_ing.p,
+                                                         cell_tot, vol.p, wi_all.p);
+    RG_HIP(hipGetLastError());
     tm.mark();  // 1
 
     // ---- start states
     DevBuf<int32_t> start, end, count, hist, counters;
-    RG_TRY(start.alloc(B.nchunks, s));
-    RG_TRY(end.alloc(B.nchunks, s));
-    RG_TRY(count.alloc(B.nchunks, s));
-    RG_TRY(hist.alloc(4 * n_in, s));
+    RG_TRY(start.alloc(chunk_tot, s));
+    RG_TRY(end.alloc(chunk_tot, s));
+    RG_TRY(count.alloc(chunk_tot, s));
+    RG_TRY(hist.alloc(4 * cell_tot, s));
     // anomalies, fixups, failed/overflow, long runs, restaged chunks
     RG_TRY(counters.alloc(8, s));
-    RG_TRY(fill_zero(hist.p, 4 * n_in, s));
+    RG_TRY(fill_zero(hist.p, 4 * cell_tot, s));
     RG_TRY(fill_zero(counters.p, 8, s));
-    k_row_start<<<grid_for(B.nrows_total * 64, 256), 256, 0, s>>>(
-        B, start.p, counters.p + 0);
+    k_row_start<<<grid_for(row_tot * 64, 256), 256, 0, s>>>(
+        d_builds, pair_of_row.p, row_tot, start.p, counters.p + 0);
     RG_HIP(hipGetLastError());
-    k_chunk_start<<<grid_for(B.nthreads, block), block, 0, s>>>(B, start.p);
+    k_chunk_start<<<grid_for(thr_tot, block), block, 0, s>>>(d_builds, pair_of_wave.p,
+                                                           thr_tot, start.p);
     RG_HIP(hipGetLastError());
     tm.mark();  // 2
 
     // ---- count (staging the triples)
     const int64_t cap = K * stage_per_edge;
     DevBuf<Triple> stage;
-    RG_TRY(stage.alloc((size_t)B.nthreads * cap, s));
-    k_walk_count<<<grid_for(B.nthreads, block), block, 0, s>>>(
-        B, start.p, end.p, count.p, hist.p, stage.p, cap);
+    RG_TRY(stage.alloc((size_t)thr_tot * cap, s));
+    k_walk_count<<<grid_for(thr_tot, block), block, 0, s>>>(
+        d_builds, pair_of_wave.p, thr_tot, start.p, end.p, count.p, hist.p,
+        stage.p, cap);
     RG_HIP(hipGetLastError());
     tm.mark();  // 3
 
     // ---- fix-up
-    k_fixup<<<grid_for(B.nrows_total, 64), 64, 0, s>>>(
-        B, start.p, end.p, count.p, hist.p, counters.p + 1, counters.p + 2,
-        stage.p, cap);
+    k_fixup<<<grid_for(row_tot, 64), 64, 0, s>>>(
+        d_builds, pair_of_row.p, row_tot, start.p, end.p, count.p, hist.p,
+        counters.p + 1, counters.p + 2, stage.p, cap);
     RG_HIP(hipGetLastError());
     DevBuf<int64_t> chunk_off, bucket_off;
-    RG_TRY(chunk_off.alloc(B.nchunks + 1, s));
-    RG_TRY(bucket_off.alloc(4 * n_in + 1, s));
-    RG_TRY(scan_i32_to_i64(count.p, chunk_off.p, B.nchunks, s));
-    RG_TRY(scan_i32_to_i64(hist.p, bucket_off.p, 4 * n_in, s));
+    RG_TRY(chunk_off.alloc(chunk_tot + 1, s));
+    RG_TRY(bucket_off.alloc(4 * cell_tot + 1, s));
+    RG_TRY(scan_i32_to_i64(count.p, chunk_off.p, chunk_tot, s));
+    RG_TRY(scan_i32_to_i64(hist.p, bucket_off.p, 4 * cell_tot, s));
     int64_t tot[2];
     int32_t cnt[4];
-    RG_TRY(read_scalars(chunk_off.p + B.nchunks, &tot[0], sizeof(int64_t), s));
-    RG_TRY(read_scalars(bucket_off.p + 4 * n_in, &tot[1], sizeof(int64_t), s));
+    RG_TRY(read_scalars(chunk_off.p + chunk_tot, &tot[0], sizeof(int64_t), s));
+    RG_TRY(read_scalars(bucket_off.p + 4 * cell_tot, &tot[1], sizeof(int64_t), s));
     RG_TRY(read_scalars(counters.p, cnt, sizeof(cnt), s));
+    up.pending = false;  // the stream has passed the upload
     tm.mark();  // 4
     if (cnt[2] != 0)
         return fail(RG_ERR_GEOMETRY,
@@ -957,20 +1301,20 @@ int build_conservative_2d(const double* xi, const double* yi, int64_t nxi,
                                     " != histogram total " +
                                     std::to_string(tot[1]));
     if (T >= ((int64_t)1 << 32))
-        return fail(RG_ERR_ARGUMENT, "more than 2^32 triples in one element");
+        return fail(RG_ERR_ARGUMENT, "more than 2^32 triples in one build");
 
     // ---- emit
     DevBuf<KV> rec;
     RG_TRY(rec.alloc(T, s));
     uint32_t* cursor = (uint32_t*)hist.p;  // reused: one word per bucket
-    k_cursor_init<<<grid_for(4 * n_in, 256), 256, 0, s>>>(bucket_off.p, cursor,
-                                                          4 * n_in);
-    k_scatter<<<grid_for(B.nthreads, block), block, 0, s>>>(
-        B, start.p, count.p, stage.p, cap, chunk_off.p, cursor, rec.p,
-        (uint32_t)T, counters.p + 2, counters.p + 4);
+    k_cursor_init<<<grid_for(4 * cell_tot, 256), 256, 0, s>>>(bucket_off.p, cursor,
+                                                              4 * cell_tot);
+    k_scatter<<<grid_for(thr_tot, block), block, 0, s>>>(
+        d_builds, pair_of_wave.p, thr_tot, start.p, count.p, stage.p, cap,
+        chunk_off.p, cursor, rec.p, (uint32_t)T, counters.p + 2, counters.p + 4);
     RG_HIP(hipGetLastError());
-    k_cursor_check<<<grid_for(4 * n_in, 256), 256, 0, s>>>(bucket_off.p, cursor,
-                                                           4 * n_in, counters.p + 2);
+    k_cursor_check<<<grid_for(4 * cell_tot, 256), 256, 0, s>>>(
+        bucket_off.p, cursor, 4 * cell_tot, counters.p + 2);
     RG_HIP(hipGetLastError());
     stage.reset();
     tm.mark();  // 5
@@ -978,120 +1322,166 @@ int build_conservative_2d(const double* xi, const double* yi, int64_t nxi,
     // ---- reduce -> CSR by input
     DevBuf<int32_t> nnz_row, long_list, group_mode;
     DevBuf<int64_t> row_off;
-    const int64_t n_groups = (n_in + 63) / 64;
-    RG_TRY(nnz_row.alloc(n_in, s));
-    RG_TRY(long_list.alloc(n_in, s));
+    const int64_t n_groups = cell_tot / 64;
+    RG_TRY(nnz_row.alloc(cell_tot, s));
+    RG_TRY(long_list.alloc(cell_tot, s));
     RG_TRY(group_mode.alloc(n_groups, s));
-    RG_TRY(row_off.alloc(n_in + 1, s));
+    RG_TRY(row_off.alloc(cell_tot + 1, s));
     k_reduce_groups<<<(unsigned)((n_groups + RED_WAVES - 1) / RED_WAVES),
-                      RED_WAVES * 64, 0, s>>>(bucket_off.p, rec.p, vol.p, wi,
-                                              n_in, nnz_row.p, group_mode.p,
+                      RED_WAVES * 64, 0, s>>>(bucket_off.p, rec.p, vol.p, wi_all.p,
+                                              cell_tot, nnz_row.p, group_mode.p,
                                               long_list.p, counters.p + 3);
     k_bucket_long<<<4, 64, 0, s>>>(bucket_off.p, rec.p, long_list.p,
                                    counters.p + 3, nnz_row.p);
     RG_HIP(hipGetLastError());
-    RG_TRY(scan_i32_to_i64(nnz_row.p, row_off.p, n_in, s));
-    int64_t nnz = 0;
-    RG_TRY(read_scalars(row_off.p + n_in, &nnz, sizeof(int64_t), s));
+    RG_TRY(scan_i32_to_i64(nnz_row.p, row_off.p, cell_tot, s));
+    // per-pair first entry and first triple (slot npairs: totals)
+    DevBuf<int64_t> bases;
+    RG_TRY(bases.alloc(2 * (npairs + 1), s));
+    k_pair_bases<<<grid_for(npairs + 1, 256), 256, 0, s>>>(
+        d_pcs, d_builds, npairs, row_off.p, cell_tot, chunk_off.p, chunk_tot, bases.p);
+    RG_HIP(hipGetLastError());
+    std::vector<int64_t> hbases(2 * (npairs + 1));
+    RG_TRY(read_scalars(bases.p, hbases.data(), sizeof(int64_t) * hbases.size(), s));
     int32_t cnt2[8];
     RG_TRY(read_scalars(counters.p, cnt2, sizeof(cnt2), s));
     if (cnt2[2] != 0)
         return fail(RG_ERR_HIP, "internal: the emission did not match the "
                                 "counted buckets (" + std::to_string(cnt2[2]) +
                                     " mismatches)");
-    if (nnz >= INT32_MAX)
-        return fail(RG_ERR_ARGUMENT, "more than 2^31 - 1 weights in one element");
+    const int64_t nnz = hbases[2 * npairs];
+    for (int64_t k = 0; k < npairs; k++)
+        if (hbases[2 * k + 2] - hbases[2 * k] >= INT32_MAX)
+            return fail(RG_ERR_ARGUMENT, "more than 2^31 - 1 weights in one element");
 
-    rg_weights* W = new rg_weights();
-    W->n_in = n_in;
-    W->n_out = n_out;
-    W->nnz = nnz;
-    W->shape_in[0] = nxi - 1;
-    W->shape_in[1] = nyi - 1;
-    W->shape_out[0] = nxo - 1;
-    W->shape_out[1] = nyo - 1;
-    (void)hipGetDevice(&W->device);
+    // ---- CSR by input, transpose -> CSR by output
     DevBuf<int32_t> rp_in, c_in, rp_out, c_out, hist_out;
     DevBuf<double> v_in, v_out;
     DevBuf<int64_t> off_out;
-    int st = RG_OK;
-    do {
-        if ((st = rp_in.alloc(n_in + 1, s))) break;
-        if ((st = c_in.alloc(nnz, s))) break;
-        if ((st = v_in.alloc(nnz, s))) break;
-        if ((st = rp_out.alloc(n_out + 1, s))) break;
-        if ((st = c_out.alloc(nnz, s))) break;
-        if ((st = v_out.alloc(nnz, s))) break;
-        if ((st = hist_out.alloc(n_out, s))) break;
-        if ((st = off_out.alloc(n_out + 1, s))) break;
-        if ((st = fill_zero(hist_out.p, n_out, s))) break;
-        k_csr_groups<<<grid_for((n_groups + 1) * 64, 256), 256, 0, s>>>(
-            bucket_off.p, row_off.p, group_mode.p, rec.p, n_in,
-            rp_in.p, c_in.p, v_in.p, hist_out.p);
-        if (hipGetLastError() != hipSuccess) {
-            st = fail(RG_ERR_HIP, "k_csr_groups launch failed");
-            break;
-        }
-        tm.mark();  // 6
-        // ---- transpose -> CSR by output
-        if ((st = scan_i32_to_i64(hist_out.p, off_out.p, n_out, s))) break;
-        if ((st = fill_zero(hist_out.p, n_out, s))) break;  // cursor
-        k_transpose_scatter<<<grid_for(n_in, 128), 128, 0, s>>>(
-            rp_in.p, c_in.p, v_in.p, n_in, off_out.p, hist_out.p, c_out.p,
-            v_out.p);
-        const int64_t sort_groups = n_out / 64 + 1;
-        k_rows_sort<<<(unsigned)((sort_groups + SORT_WAVES - 1) / SORT_WAVES),
-                      SORT_WAVES * 64, 0, s>>>(off_out.p, n_out, c_out.p,
-                                               v_out.p, rp_out.p);
-        if (hipGetLastError() != hipSuccess) {
-            st = fail(RG_ERR_HIP, "transpose launch failed");
-            break;
-        }
-        tm.mark();  // 7
-    } while (0);
-    if (st != RG_OK) {
-        delete W;
-        return st;
-    }
-    W->rowptr_in = rp_in.release();
-    W->col_in = c_in.release();
-    W->val_in = v_in.release();
-    W->rowptr_out = rp_out.release();
-    W->col_out = c_out.release();
-    W->val_out = v_out.release();
-    rg_build_stats& S = W->stats;
-    S.n_triples = T;
-    S.n_chunks = B.nchunks;
-    S.n_fixups = cnt[1];
-    S.n_search_anomalies = cnt[0];
-    S.n_restaged = cnt2[4];
-    // every inside sub-segment emits two triples except on the first and
-    // last row of a sweep, which emit one
-    S.n_subsegments = T / 2;
+    RG_TRY(rp_in.alloc(cell_tot + npairs, s));
+    RG_TRY(c_in.alloc(nnz, s));
+    RG_TRY(v_in.alloc(nnz, s));
+    RG_TRY(rp_out.alloc(out_tot + npairs, s));
+    RG_TRY(c_out.alloc(nnz, s));
+    RG_TRY(v_out.alloc(nnz, s));
+    RG_TRY(hist_out.alloc(out_tot, s));
+    RG_TRY(off_out.alloc(out_tot + 1, s));
+    RG_TRY(fill_zero(hist_out.p, out_tot, s));
+    k_csr_groups<<<grid_for(n_groups * 64, 256), 256, 0, s>>>(
+        d_pcs, pair_of_ing.p, n_groups, bucket_off.p, row_off.p, group_mode.p,
+        rec.p, rp_in.p, c_in.p, v_in.p, hist_out.p);
+    RG_HIP(hipGetLastError());
+    tm.mark();  // 6
+    RG_TRY(scan_i32_to_i64(hist_out.p, off_out.p, out_tot, s));
+    RG_TRY(fill_zero(hist_out.p, out_tot, s));  // cursor
+    k_transpose_scatter<<<grid_for(cell_tot, 128), 128, 0, s>>>(
+        d_pcs, pair_of_ing.p, cell_tot, row_off.p, c_in.p, v_in.p, off_out.p,
+        hist_out.p, c_out.p, v_out.p);
+    const int64_t sort_groups = out_tot / 64;
+    k_rows_sort<<<(unsigned)((sort_groups + SORT_WAVES - 1) / SORT_WAVES),
+                  SORT_WAVES * 64, 0, s>>>(d_pcs, pair_of_outg.p, off_out.p, out_tot,
+                                           c_out.p, v_out.p, rp_out.p);
+    RG_HIP(hipGetLastError());
+    tm.mark();  // 7
+
+    // ---- handles
+    std::vector<rg_weights*> made;
+    auto arena = std::make_shared<rg_arena>();
+    int32_t* a_rp_in = rp_in.release();
+    int32_t* a_c_in = c_in.release();
+    double* a_v_in = v_in.release();
+    int32_t* a_rp_out = rp_out.release();
+    int32_t* a_c_out = c_out.release();
+    double* a_v_out = v_out.release();
+    arena->bufs = {a_rp_in, a_c_in, a_v_in, a_rp_out, a_c_out, a_v_out};
+    int device = 0;
+    (void)hipGetDevice(&device);
+    rg_build_stats batch = {};
+    batch.n_fixups = cnt[1];
+    batch.n_search_anomalies = cnt[0];
+    batch.n_restaged = cnt2[4];
     if (timings) {
-        S.ms_prepare = tm.ms(0, 1);
-        S.ms_start = tm.ms(1, 2);
-        S.ms_count = tm.ms(2, 3);
-        S.ms_fixup = tm.ms(3, 4);
-        S.ms_emit = tm.ms(4, 5);
-        S.ms_reduce = tm.ms(5, 6);
-        S.ms_transpose = tm.ms(6, 7);
-        S.ms_total = tm.ms(0, 7);
+        batch.ms_prepare = tm.ms(0, 1);
+        batch.ms_start = tm.ms(1, 2);
+        batch.ms_count = tm.ms(2, 3);
+        batch.ms_fixup = tm.ms(3, 4);
+        batch.ms_emit = tm.ms(4, 5);
+        batch.ms_reduce = tm.ms(5, 6);
+        batch.ms_transpose = tm.ms(6, 7);
+        batch.ms_total = tm.ms(0, 7);
     }
-    (void)t_host0;
-    *out = W;
+    for (int64_t k = 0; k < npairs; k++) {
+        const rg_pair_desc& d = pairs[k];
+        const PairCells& P = pcs[k];
+        const int64_t eb = hbases[2 * k];
+        rg_weights* W = new rg_weights();
+        W->device = device;
+        W->n_in = P.n_in;
+        W->n_out = P.n_out;
+        W->nnz = hbases[2 * k + 2] - eb;
+        W->shape_in[0] = d.nxi - 1;
+        W->shape_in[1] = d.nyi - 1;
+        W->shape_out[0] = d.nxo - 1;
+        W->shape_out[1] = d.nyo - 1;
+        W->rowptr_in = a_rp_in + P.cell_gbase + k;
+        W->col_in = a_c_in + eb;
+        W->val_in = a_v_in + eb;
+        W->rowptr_out = a_rp_out + P.out_gbase + k;
+        W->col_out = a_c_out + eb;
+        W->val_out = a_v_out + eb;
+        W->arena = arena;
+        rg_build_stats& S = W->stats;
+        S = batch;
+        S.n_triples = hbases[2 * k + 3] - hbases[2 * k + 1];
+        S.n_chunks = builds[k].nchunks;
+        // every inside sub-segment emits two triples except on the first and
+        // last row of a sweep, which emit one
+        S.n_subsegments = S.n_triples / 2;
+        out[k] = W;
+    }
     return RG_OK;
 }
+
+int build_conservative_2d(const double* xi, const double* yi, int64_t nxi,
+                          int64_t nyi, const double* xo, const double* yo,
+                          int64_t nxo, int64_t nyo, const double* wi,
+                          const rg_tuning* tuning, rg_weights** out,
+                          hipStream_t s)
+{
+    rg_pair_desc d;
+    d.xi = xi;
+    d.yi = yi;
+    d.nxi = nxi;
+    d.nyi = nyi;
+    d.xo = xo;
+    d.yo = yo;
+    d.nxo = nxo;
+    d.nyo = nyo;
+    d.wi = wi;
+    if (!out) return fail(RG_ERR_ARGUMENT, "out is NULL");
+    return build_batched(&d, 1, tuning, out, s);
+}
+
+}  // namespace rg
+
+rg_arena::~rg_arena()
+{
+    for (void* p : bufs) rg::dev_free(p);
+}
+
+namespace rg {
 
 void weights_release(rg_weights* W)
 {
     if (!W) return;
-    dev_free(W->rowptr_in);
-    dev_free(W->col_in);
-    dev_free(W->val_in);
-    dev_free(W->rowptr_out);
-    dev_free(W->col_out);
-    dev_free(W->val_out);
+    if (!W->arena) {
+        dev_free(W->rowptr_in);
+        dev_free(W->col_in);
+        dev_free(W->val_in);
+        dev_free(W->rowptr_out);
+        dev_free(W->col_out);
+        dev_free(W->val_out);
+    }
     dev_free(W->val_in_f32);
     dev_free(W->val_out_f32);
     for (auto& pl : W->plan) {
@@ -1293,6 +1683,75 @@ int rg_build_conservative_2d(const double* xi, const double* yi, int64_t nxi,
     if (!out) return fail(RG_ERR_ARGUMENT, "out is NULL");
     return build_conservative_2d(xi, yi, nxi, nyi, xo, yo, nxo, nyo, wi, tuning,
                                  out, (hipStream_t)stream);
+}
+
+int rg_build_conservative_2d_batched(const rg_pair_desc* pairs, int64_t npairs,
+                                     const rg_tuning* tuning, rg_weights_t* out,
+                                     void* stream)
+{
+    return build_batched(pairs, npairs, tuning, out, (hipStream_t)stream);
+}
+
+int rg_tune_build(const rg_pair_desc* pairs, int64_t npairs,
+                  const int32_t* edges_per_lane, int n_epl,
+                  const int32_t* block, int n_block, int reps,
+                  const rg_tuning* base, rg_tuning* best, double* best_ms,
+                  double* times, void* stream)
+{
+    if (!edges_per_lane || !block || n_epl <= 0 || n_block <= 0 || !best)
+        return fail(RG_ERR_ARGUMENT, "empty candidate list or NULL output");
+    if (reps < 1) reps = 1;
+    if (reps > 64) reps = 64;
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<rg_weights*> hs((size_t)npairs > 0 ? (size_t)npairs : 1, nullptr);
+    hipEvent_t e0, e1;
+    RG_HIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(RG_ERR_HIP, "hipEventCreate failed");
+    }
+    double top = -1;
+    int st = RG_OK;
+    for (int a = 0; a < n_epl && st == RG_OK; a++) {
+        for (int b = 0; b < n_block && st == RG_OK; b++) {
+            rg_tuning t = base ? *base : rg_tuning{};
+            t.edges_per_lane = edges_per_lane[a];
+            t.block = block[b];
+            t.flags &= ~RG_TUNE_TIMINGS;
+            std::vector<double> ms;
+            for (int r = 0; r <= reps && st == RG_OK; r++) {
+                if (hipEventRecord(e0, s) != hipSuccess) st = fail(RG_ERR_HIP, "event record");
+                if (st == RG_OK) st = build_batched(pairs, npairs, &t, hs.data(), s);
+                if (st == RG_OK && hipEventRecord(e1, s) != hipSuccess)
+                    st = fail(RG_ERR_HIP, "event record");
+                for (auto& h : hs) {
+                    weights_release(h);
+                    h = nullptr;
+                }
+                if (st != RG_OK) break;
+                float f = 0;
+                if (hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&f, e0, e1) != hipSuccess) {
+                    st = fail(RG_ERR_HIP, "event timing");
+                    break;
+                }
+                if (r > 0) ms.push_back(f);  // the first run is untimed
+            }
+            if (st != RG_OK) break;
+            std::sort(ms.begin(), ms.end());
+            const double med = ms[ms.size() / 2];
+            if (times) times[a * n_block + b] = med;
+            if (top < 0 || med < top) {
+                top = med;
+                *best = t;
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st != RG_OK) return st;
+    if (best_ms) *best_ms = top;
+    return RG_OK;
 }
 
 int rg_weights_info(rg_weights_t w, int64_t* n_in, int64_t* n_out,

@@ -7,7 +7,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <memory>
 #include <string>
+#include <vector>
 
 #include "../../include/regridding_amd.h"
 
@@ -101,6 +103,27 @@ inline unsigned grid_for(int64_t n, int block)
 // twice on the device.  CSR by input cell (columns = output cells ascending)
 // is the reference's storage order (sorted by (input, output)); CSR by
 // output cell (columns = input cells ascending) drives the forward apply.
+// Device buffers shared by the handles of one batched build: the handles'
+// CSR arrays are slices of them, freed with the last handle.
+namespace rg {
+// Largest k in [0, n) with base[k] <= x (base ascending, base[0] <= x).
+__device__ inline int seg_of(const int64_t* __restrict__ base, int n, int64_t x)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (base[mid] <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+}  // namespace rg
+
+struct rg_arena {
+    std::vector<void*> bufs;
+    ~rg_arena();
+};
+
 struct rg_weights {
     int device = 0;
     int64_t n_in = 0, n_out = 0, nnz = 0;
@@ -124,4 +147,6 @@ struct rg_weights {
         int64_t max_cells = 0;     // largest patch (cells)
     } plan[2];
     rg_build_stats stats = {};
+    // set when the six CSR arrays are slices of a batched build's buffers
+    std::shared_ptr<rg_arena> arena;
 };

@@ -78,8 +78,13 @@ struct Build {
     int64_t K;               // edges per chunk
     int64_t nchunks;
     int64_t nrows_total;
-    int64_t row_base[4];     // global id of the first row of each pass
+    int64_t row_base[4];     // id of the first row of each pass
     int64_t nthreads;        // walk threads (a multiple of 64 per pass)
+    // Where this grid pair sits in a batched build's concatenated index
+    // spaces (all 0 for a single pair).  The ids above are pair-local.
+    // Input and output cells are padded to multiples of 64 per pair, so
+    // every group of 64 cells belongs to one pair.
+    int64_t thread_gbase, chunk_gbase, row_gbase, cell_gbase, out_gbase;
 };
 
 // Walk threads and chunks.  Chunk ids are canonical -- pass, row, chunk

@@ -15,7 +15,7 @@ import pytest
 from oracle import oracle
 from regridding_amd import _util, workloads
 from regridding_amd._conservative import build_2d
-from regridding_amd._lib import DeviceArray, check, lib
+from regridding_amd._lib import DeviceArray, check, lib, rg_apply_job
 from regridding_amd._regrid import import_weights
 
 pytestmark = pytest.mark.gpu
@@ -34,6 +34,9 @@ CASES = {
                                workloads.warped_grid(12, 0.05, 0.0, 0.0, -0.8, 0.8)),
     "disjoint_partial": lambda: (workloads.warped_grid((40, 31), 0.3, 0.05, 0.05),
                                  workloads.warped_grid((27, 35), -0.2, 0.01, 0.0, 0.2, 1.6)),
+    # input cells far larger than output cells: buckets beyond the LDS share
+    "very_coarse_to_fine": lambda: (workloads.warped_grid(5, 0.2, 0.02, 0.02),
+                                    workloads.warped_grid(150, 0.4, 0.0, 0.0, -0.9, 0.9)),
     "output_covers_input": lambda: (workloads.warped_grid(20, 0.1, 0.02, 0.02, -0.5, 0.5),
                                     workloads.warped_grid(30, 0.3, 0.0, 0.0, -1.5, 1.5)),
 }
@@ -141,8 +144,10 @@ def test_apply_matches_oracle(D):
 @pytest.mark.parametrize("ipp,lds_kib", [(1, 32), (4, 1), (16, 64)])
 def test_apply_launch_shapes_same_bits(tile_w, ipp, lds_kib):
     # planned (LDS-staged) and direct gathers, any tile and image grouping:
-    # the per-output accumulation order never changes
-    ci, co = _perturbed(*CASES["coarse_to_fine"]())
+    # the per-output accumulation order never changes (the output grid has
+    # more than 65536 cells, so the staged path is taken)
+    ci, co = _perturbed(workloads.warped_grid(330, 0.3, 0.05, 0.04),
+                        workloads.warped_grid(300, 0.1, 0.0, 0.0, -0.9, 0.9))
     h = build_2d(ci, co)
     ii, io, w = h.export()
     x = np.random.default_rng(4).normal(size=(7, h.n_in))
@@ -155,6 +160,29 @@ def test_apply_launch_shapes_same_bits(tile_w, ipp, lds_kib):
         assert np.array_equal(_apply(h, y, transpose=True), ref_t)
     finally:
         check(lib().rg_set_apply_tuning(16, 8, 64))
+
+
+def test_apply_batched_matches_single():
+    pairs = [CASES[c]() for c in sorted(CASES)]
+    hs, xs, ys, refs = [], [], [], []
+    jobs = (rg_apply_job * (2 * len(pairs)))()
+    for k, (gi, go) in enumerate(pairs):
+        ci, co = _perturbed(gi, go)
+        h = build_2d(ci, co)
+        hs.append(h)
+        for t in (0, 1):
+            D = 1 + (3 * k + t) % 11
+            n_src, n_dst = (h.n_in, h.n_out) if t == 0 else (h.n_out, h.n_in)
+            x = DeviceArray.from_host(np.random.default_rng(k).normal(size=(D, n_src)))
+            y = DeviceArray.empty((D, n_dst), np.float64)
+            refs.append(_apply(h, x.to_host(), transpose=bool(t)))
+            j = jobs[2 * k + t]
+            j.weights, j.transpose, j.values_input, j.values_output, j.batch = h.ptr, t, x.ptr, y.ptr, D
+            xs.append(x)
+            ys.append(y)
+    check(lib().rg_apply_batched_f64(jobs, len(jobs), 0))
+    for y, ref in zip(ys, refs):
+        assert np.array_equal(y.to_host(), ref)
 
 
 def test_apply_f32_tolerance():
