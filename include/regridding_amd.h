@@ -76,6 +76,9 @@ int rg_free(void* ptr);
 int rg_memcpy_htod(void* dst, const void* src, size_t bytes, void* stream);
 int rg_memcpy_dtoh(void* dst, const void* src, size_t bytes, void* stream);
 int rg_memset(void* dst, int value, size_t bytes, void* stream);
+/* Pinned (page-locked) host memory, for asynchronous transfers. */
+int rg_host_alloc(void** ptr, size_t bytes);
+int rg_host_free(void* ptr);
 int rg_stream_create(void** stream);
 int rg_stream_destroy(void* stream);
 int rg_stream_sync(void* stream);
@@ -244,6 +247,73 @@ int rg_find_indices_1d(const double* x_input, int64_t m,
  * named site ("build", "apply", "alloc") fail with RG_ERR_INJECTED.  NULL
  * clears pending injections. */
 int rg_inject_failure(const char* site);
+
+/* ---- serving loop (no reference counterpart) ------------------------------
+ *
+ * A request is one regrid job: a grid pair (or the cache key of weights
+ * built earlier) and a stack of value arrays.  Submitted requests queue in
+ * arrival order; worker threads, each with its own stream, repeatedly take
+ * the longest prefix of the queue that fits the free device memory (each
+ * request's footprint is estimated from its sizes), build the weights of
+ * all cache misses of that group in one batched build, apply every request
+ * of the group in one batched apply, and copy the results back.  Groups of
+ * different workers overlap on the device; a request that arrives while
+ * others run joins the next group a worker forms.  Built weights stay on the
+ * device, keyed by the caller's grid hash (LRU, bounded by cache_bytes). */
+
+#define RG_F64 0
+#define RG_F32 1
+
+typedef struct rg_serve_config {
+    int32_t workers;       /* groups in flight, one stream each (default 3) */
+    int32_t max_group;     /* most requests in one group (default 64) */
+    int64_t reserve_bytes; /* device memory admission leaves free (4 GiB) */
+    int64_t cache_bytes;   /* weight cache capacity (default 32 GiB) */
+    rg_tuning tuning;      /* tuning of the batched builds */
+} rg_serve_config;
+
+typedef struct rg_serve_request {
+    uint64_t key;      /* weight-cache key (equal keys: equal weights); 0: not cached */
+    const double* xi;  /* host vertex grids, already perturbed (read on a miss) */
+    const double* yi;
+    int64_t nxi, nyi;
+    const double* xo;
+    const double* yo;
+    int64_t nxo, nyo;
+    const double* wi;       /* host input weighting or NULL */
+    int32_t dtype;          /* RG_F64 or RG_F32 */
+    int32_t transpose;      /* 1: apply the adjoint (output cells -> input cells) */
+    const void* values;     /* host (batch, n_in), or (batch, n_out) if transpose */
+    void* result;           /* host (batch, n_out), or (batch, n_in) if transpose */
+    int64_t batch;
+    int64_t arrival_ns;     /* rg_clock_ns() time of arrival (0: at submission) */
+} rg_serve_request;
+
+typedef struct rg_serve_stats {
+    int64_t submitted, completed, failed;
+    int64_t cache_hits, cache_misses, builds, groups;
+    int64_t max_group_seen;
+    int64_t peak_admitted_bytes;
+    int64_t cache_entries, cache_bytes;
+} rg_serve_stats;
+
+typedef struct rg_server* rg_server_t;
+
+/* Monotonic clock (ns) of arrival and completion times. */
+int64_t rg_clock_ns(void);
+int rg_serve_create(const rg_serve_config* config, rg_server_t* out);
+/* Thread-safe.  The request's host buffers must stay valid until
+ * rg_serve_wait returns for its ticket. */
+int rg_serve_submit(rg_server_t server, const rg_serve_request* request,
+                    uint64_t* ticket);
+/* Blocks until the request is done; returns its status (rg_last_error holds
+ * its message on failure) and its completion time.  Each ticket is waited
+ * for once. */
+int rg_serve_wait(rg_server_t server, uint64_t ticket, int64_t* done_ns);
+int rg_serve_stats_get(rg_server_t server, rg_serve_stats* stats);
+/* Finishes every submitted request, then stops the workers and frees the
+ * cache. */
+int rg_serve_destroy(rg_server_t server);
 
 #ifdef __cplusplus
 }

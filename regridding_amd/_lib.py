@@ -73,6 +73,43 @@ class rg_apply_job(ctypes.Structure):
     ]
 
 
+class rg_serve_config(ctypes.Structure):
+    _fields_ = [
+        ("workers", ctypes.c_int32),
+        ("max_group", ctypes.c_int32),
+        ("reserve_bytes", ctypes.c_int64),
+        ("cache_bytes", ctypes.c_int64),
+        ("tuning", rg_tuning),
+    ]
+
+
+class rg_serve_request(ctypes.Structure):
+    _fields_ = [
+        ("key", ctypes.c_uint64),
+        ("xi", ctypes.c_void_p),
+        ("yi", ctypes.c_void_p),
+        ("nxi", ctypes.c_int64),
+        ("nyi", ctypes.c_int64),
+        ("xo", ctypes.c_void_p),
+        ("yo", ctypes.c_void_p),
+        ("nxo", ctypes.c_int64),
+        ("nyo", ctypes.c_int64),
+        ("wi", ctypes.c_void_p),
+        ("dtype", ctypes.c_int32),
+        ("transpose", ctypes.c_int32),
+        ("values", ctypes.c_void_p),
+        ("result", ctypes.c_void_p),
+        ("batch", ctypes.c_int64),
+        ("arrival_ns", ctypes.c_int64),
+    ]
+
+
+class rg_serve_stats(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_int64) for name in (
+        "submitted", "completed", "failed", "cache_hits", "cache_misses", "builds",
+        "groups", "max_group_seen", "peak_admitted_bytes", "cache_entries", "cache_bytes")]
+
+
 class rg_build_stats(ctypes.Structure):
     _fields_ = [
         ("n_triples", ctypes.c_int64),
@@ -135,6 +172,15 @@ SIGNATURES = {
          ctypes.POINTER(rg_tuning), ctypes.POINTER(rg_tuning), ctypes.POINTER(ctypes.c_double),
          _vp, _vp],
     ),
+    "rg_host_alloc": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_size_t]),
+    "rg_host_free": (ctypes.c_int, [_vp]),
+    "rg_clock_ns": (ctypes.c_int64, []),
+    "rg_serve_create": (ctypes.c_int, [ctypes.POINTER(rg_serve_config), ctypes.POINTER(_vp)]),
+    "rg_serve_submit": (ctypes.c_int, [_vp, ctypes.POINTER(rg_serve_request),
+                                       ctypes.POINTER(ctypes.c_uint64)]),
+    "rg_serve_wait": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int64)]),
+    "rg_serve_stats_get": (ctypes.c_int, [_vp, ctypes.POINTER(rg_serve_stats)]),
+    "rg_serve_destroy": (ctypes.c_int, [_vp]),
     "rg_weights_info": (ctypes.c_int, [_vp, _pi64, _pi64, _pi64]),
     "rg_weights_stats": (ctypes.c_int, [_vp, ctypes.POINTER(rg_build_stats)]),
     "rg_weights_export": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),

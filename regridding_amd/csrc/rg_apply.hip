@@ -482,6 +482,8 @@ int ensure_f32(rg_weights* W, hipStream_t s)
     k_to_f32<<<gs, 256, 0, s>>>(W->val_in, a, W->nnz);
     k_to_f32<<<gs, 256, 0, s>>>(W->val_out, b, W->nnz);
     RG_HIP(hipGetLastError());
+    // other streams may use the copies as soon as they are published
+    RG_HIP(hipStreamSynchronize(s));
     W->val_in_f32 = a;
     W->val_out_f32 = b;
     return RG_OK;

@@ -17,6 +17,7 @@ __all__ = [
     "plain_small",
     "plain_large",
     "batched_pairs",
+    "inversion_problem",
 ]
 
 
@@ -84,3 +85,30 @@ def batched_pairs(n_pairs: int = 256, seed: int = 3, n_min: int = 33, n_max: int
             )
         )
     return pairs
+
+
+def inversion_problem(seed: int = 5, n_truth: int = 1025, n_obs: int = 769, n_views: int = 8):
+    """The adjoint-inversion workload: a truth image on the rectilinear
+    ``linspace(-1, 1, n_truth)`` vertex grid -- 50 Gaussian blobs (centres
+    U(-0.8, 0.8), widths U(0.02, 0.1), amplitudes U(50, 500)) on a flat
+    background of 10, sampled at the cell centres -- and `n_views`
+    observation grids of ``n_obs`` vertices per axis over [-0.9, 0.9], each
+    with its own rotation U(0, pi/4) and warp amplitudes U(0.01, 0.08).
+    Returns ``(truth_grid, truth_image, [obs_grid, ...], rng)``; the rng
+    continues the stream for the Poisson draws of the observations."""
+    rng = np.random.default_rng(seed)
+    grid = warped_grid(n_truth)
+    xc = 0.25 * (grid[0][:-1, :-1] + grid[0][1:, :-1] + grid[0][:-1, 1:] + grid[0][1:, 1:])
+    yc = 0.25 * (grid[1][:-1, :-1] + grid[1][1:, :-1] + grid[1][:-1, 1:] + grid[1][1:, 1:])
+    image = np.full(xc.shape, 10.0)
+    centres = rng.uniform(-0.8, 0.8, (50, 2))
+    widths = rng.uniform(0.02, 0.1, 50)
+    amps = rng.uniform(50, 500, 50)
+    for (cx, cy), w, a in zip(centres, widths, amps):
+        image += a * np.exp(-((xc - cx) ** 2 + (yc - cy) ** 2) / (2 * w * w))
+    views = []
+    for _ in range(n_views):
+        theta = rng.uniform(0, np.pi / 4)
+        ax, ay = rng.uniform(0.01, 0.08, 2)
+        views.append(warped_grid(n_obs, theta, ax, ay, -0.9, 0.9))
+    return grid, image, views, rng

@@ -21,7 +21,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def _declared():
     text = open(os.path.join(ROOT, "include", "regridding_amd.h")).read()
     # function declarations: a return type, the name, an opening parenthesis
-    decl = r"^(?:const\s+char\s*\*|int|void)\s*(rg_[a-z0-9_]+)\s*\("
+    decl = r"^(?:const\s+char\s*\*|int64_t|int|void)\s*(rg_[a-z0-9_]+)\s*\("
     return sorted(set(re.findall(decl, text, flags=re.M)))
 
 
