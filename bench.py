@@ -57,6 +57,10 @@ def parse_args():
     p.add_argument("--no-batched", action="store_true",
                    help="skip the batched-vs-plain comparison over 256 mixed pairs")
     p.add_argument("--batched-pairs", type=int, default=256)
+    p.add_argument("--no-serve", action="store_true", help="skip the serving-loop run")
+    p.add_argument("--serve-seconds", type=float, default=30.0)
+    p.add_argument("--serve-rate", type=float, default=2000.0, help="request arrivals per second")
+    p.add_argument("--no-inversion", action="store_true", help="skip the adjoint inversion run")
     return p.parse_args()
 
 
@@ -260,6 +264,10 @@ def main() -> int:
     del dx32, dy32, dt64
     if not args.no_batched:
         result["batched"] = bench_batched(args)
+    if not args.no_serve:
+        result["serving"] = bench_serve(args)
+    if not args.no_inversion:
+        result["inversion"] = bench_inversion(args)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(ci, co, values, build())
     print(json.dumps(result), flush=True)
@@ -379,6 +387,218 @@ def bench_batched(args) -> dict:
         "batch_build_phases_ms": {k[3:]: round(v, 3) for k, v in st.items() if k.startswith("ms_")},
         "batch_autotune": tuned,
     }
+
+
+def bench_serve(args) -> dict:
+    """The serving loop under a Poisson arrival stream: 60% of requests
+    reuse one of 64 grid pairs whose weights are cached on the device, 40%
+    bring a pair that is built on arrival (drawn from a pool of 128
+    pre-generated pairs and submitted uncached, so each one is a full
+    build); per-axis vertices log-uniform in [33, 1025]; 1-32 value arrays
+    per request, FP32 or FP64 50/50, drawn as slices of a pinned pool of
+    Poisson counts.  Latency is scheduled arrival -> result in host memory."""
+    import threading
+    from collections import deque
+
+    from regridding_amd import _util, workloads
+    from regridding_amd.serve import Server, clock_ns, grid_key, pinned_empty
+
+    rng = np.random.default_rng(4)
+    t0 = time.perf_counter()
+
+    def make(n, seed):
+        out = []
+        for p in workloads.batched_pairs(n_pairs=n, seed=seed):
+            ci, co, *_ = _util.normalize_coordinates(p["grid_input"], p["grid_output"], perturb=True)
+            out.append((ci, co))
+        return out
+
+    cached = make(64, 40)
+    fresh = make(128, 41)
+    keys = [grid_key(ci, co) for ci, co in cached]
+    pool_n = 40 << 20
+    pool64 = pinned_empty((pool_n,), np.float64)
+    pool64[:] = rng.poisson(20.0, pool_n)
+    pool32 = pinned_empty((pool_n,), np.float32)
+    pool32[:] = pool64
+    ring_bytes = 6 << 30
+    ring = pinned_empty((ring_bytes,), np.uint8)
+    log(f"serving setup {time.perf_counter() - t0:.1f} s")
+
+    def cells(g):
+        return (g[0].shape[0] - 1) * (g[0].shape[1] - 1)
+
+    # arrivals and request shapes, fixed in advance
+    n_req = int(args.serve_rate * args.serve_seconds * 1.2) + 16
+    gaps = rng.exponential(1.0 / args.serve_rate, n_req)
+    arrive = np.cumsum(gaps)
+    arrive = arrive[arrive < args.serve_seconds]
+    plan = []
+    for _ in range(arrive.size):
+        reuse = rng.random() < 0.6
+        j = int(rng.integers(64 if reuse else 128))
+        B = int(rng.integers(1, 33))
+        f32 = bool(rng.random() < 0.5)
+        plan.append((reuse, j, B, f32))
+
+    srv = Server(workers=3, max_group=64)
+    # warm the cache with the 64 reusable pairs (not timed)
+    warm = []
+    for j, (ci, co) in enumerate(cached):
+        x = pool64[: cells(ci)].reshape(1, -1)
+        warm.append(srv.submit(x, ci, co, key=keys[j]))
+    for tk in warm:
+        srv.wait(tk)
+
+    lat = []
+    inflight = deque()
+    lock = threading.Condition()
+    live = deque()  # result slices of the ring in submission order
+    state = {"done": False}
+
+    def ring_take(n):
+        # a contiguous slice of the pinned result ring; slices are released
+        # in submission order (the completer waits in that order)
+        with lock:
+            while True:
+                if not live:
+                    start = 0 if n <= ring_bytes else None
+                else:
+                    head = live[-1][0] + live[-1][1]
+                    tail = live[0][0]
+                    if live[-1][0] >= tail:  # not wrapped: free [head, end) and [0, tail)
+                        start = head if head + n <= ring_bytes else (0 if n <= tail else None)
+                    else:  # wrapped: free [head, tail)
+                        start = head if head + n <= tail else None
+                if start is None and not live:
+                    raise RuntimeError("a result does not fit the result ring")
+                if start is not None:
+                    live.append((start, n))
+                    return start
+                lock.wait()
+
+    def ring_give():
+        with lock:
+            live.popleft()
+            lock.notify_all()
+
+    def completer():
+        while True:
+            with lock:
+                while not inflight and not state["done"]:
+                    lock.wait()
+                if not inflight and state["done"]:
+                    return
+                tk = inflight.popleft()
+            _, done_ns = srv.wait(tk)
+            lat.append((done_ns - tk.arrival_ns) * 1e-6)
+            ring_give()
+
+    th = threading.Thread(target=completer)
+    th.start()
+    base = clock_ns() + 50_000_000
+    stats0 = srv.stats()
+    for (reuse, j, B, f32), t in zip(plan, arrive):
+        ci, co = cached[j] if reuse else fresh[j]
+        n_in, n_out = cells(ci), cells(co)
+        pool = pool32 if f32 else pool64
+        es = 4 if f32 else 8
+        off = int(rng.integers(0, pool_n - B * n_in + 1))
+        x = pool[off: off + B * n_in].reshape(B, n_in)
+        start = ring_take(B * n_out * es)
+        res = ring[start: start + B * n_out * es].view(np.float32 if f32 else np.float64).reshape(B, n_out)
+        at = base + int(t * 1e9)
+        while clock_ns() < at:
+            time.sleep(max(0.0, (at - clock_ns()) * 1e-9 - 2e-4))
+        if reuse:
+            tk = srv.submit(x, key=keys[j], shapes=(ci[0].shape, co[0].shape), result=res, arrival_ns=at)
+        else:
+            tk = srv.submit(x, ci, co, key=0, result=res, arrival_ns=at)
+        with lock:
+            inflight.append(tk)
+            lock.notify_all()
+    with lock:
+        state["done"] = True
+        lock.notify_all()
+    th.join()
+    end_ns = clock_ns()
+    st = srv.stats()
+    srv.close()
+    lat = np.asarray(lat)
+    wall = (end_ns - base) * 1e-9
+    n_done = st["completed"] - stats0["completed"]
+    out = {
+        "workload": f"Poisson arrivals at {args.serve_rate:.0f}/s for {args.serve_seconds:.0f} s; 60% reuse "
+                    "one of 64 device-cached pairs, 40% build on arrival; per-axis vertices log-uniform "
+                    "in [33,1025]; 1-32 value arrays, FP32/FP64 50/50",
+        "requests": int(lat.size),
+        "p50_ms": round(float(np.percentile(lat, 50)), 3),
+        "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "mean_ms": round(float(lat.mean()), 3),
+        "throughput_req_per_s": round(n_done / wall, 1),
+        "offered_req_per_s": round(lat.size / args.serve_seconds, 1),
+        "builds": st["builds"] - stats0["builds"],
+        "cache_hits": st["cache_hits"] - stats0["cache_hits"],
+        "groups": st["groups"] - stats0["groups"],
+        "max_group": st["max_group_seen"],
+        "peak_admitted_GB": round(st["peak_admitted_bytes"] / 1e9, 3),
+        "failed": st["failed"],
+    }
+    log(f"serving: {out['requests']} requests, p50 {out['p50_ms']} ms, p99 {out['p99_ms']} ms, "
+        f"{out['throughput_req_per_s']} req/s, {out['groups']} groups, {out['builds']} builds")
+    return out
+
+
+def bench_inversion(args) -> dict:
+    """Adjoint inversion: a 1024x1024-cell image (50 Gaussian blobs + flat
+    background) seen through 8 warped 769x769-vertex observation grids with
+    Poisson noise; 200 iterations of accelerated gradient least squares whose
+    gradient is torch.autograd through the transpose-apply kernel."""
+    import torch
+
+    from regridding_amd import _util, workloads
+    from regridding_amd._conservative import build_2d_batched
+    from regridding_amd.inversion import least_squares, lipschitz
+    from regridding_amd.torch_ops import regrid_apply
+
+    grid, image, views, rng = workloads.inversion_problem()
+    pairs = []
+    for v in views:
+        ci, co, *_ = _util.normalize_coordinates(grid, v, perturb=True)
+        pairs.append((ci, co))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hs = build_2d_batched(pairs)
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t0
+    truth = torch.tensor(image.ravel(), device="cuda")
+    obs = []
+    for w in hs:
+        lam = regrid_apply(w, truth).cpu().numpy()
+        obs.append(torch.tensor(rng.poisson(np.maximum(lam, 0)).astype(np.float64), device="cuda"))
+    vt = list(zip(hs, obs))
+    L = lipschitz(vt, truth.numel(), device="cuda")
+    x0 = torch.full_like(truth, float(image.mean()))
+    least_squares(vt, x0, iters=2, step=1.0 / L)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x, losses = least_squares(vt, x0, iters=200, step=1.0 / L)
+    torch.cuda.synchronize()
+    t_iter = (time.perf_counter() - t0) / 200
+    err0 = float(torch.linalg.norm(x0 - truth) / torch.linalg.norm(truth))
+    err = float(torch.linalg.norm(x - truth) / torch.linalg.norm(truth))
+    out = {
+        "workload": "1024x1024-cell truth, 8 warped 769x769-vertex views, Poisson noise, 200 iterations",
+        "build_ms_8_views_batched": round(t_build * 1e3, 3),
+        "ms_per_iteration": round(t_iter * 1e3, 3),
+        "loss_first": losses[0],
+        "loss_last": losses[-1],
+        "relative_error_start": round(err0, 5),
+        "relative_error_end": round(err, 5),
+    }
+    log(f"inversion: build {out['build_ms_8_views_batched']} ms, {out['ms_per_iteration']} ms/iteration, "
+        f"loss {losses[0]:.4g} -> {losses[-1]:.4g}, relative error {err0:.4f} -> {err:.4f}")
+    return out
 
 
 def cpu_baseline(ci, co, values, handle) -> dict:

@@ -199,6 +199,20 @@ _lock = threading.Lock()
 _lib = None
 
 
+def _load_torch_runtime_first() -> None:
+    """PyTorch-ROCm carries its own HIP runtime under the soname this
+    library links (libamdhip64.so.7), and a process can hold only one: the
+    first loaded serves both.  When torch is installed it is imported before
+    the library, so torch tensors and this library share torch's runtime (a
+    torch imported after the ROCm one finds no GPU)."""
+    if os.environ.get("RG_NO_TORCH"):
+        return
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     """Load the shared library and declare its signatures (no GPU call)."""
     if not os.path.exists(path):
@@ -206,6 +220,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
             f"{path} is missing: the HIP extension of regridding_amd is not "
             "built (run __graft_entry__.build() or make -C regridding_amd/csrc)"
         )
+    _load_torch_runtime_first()
     L = ctypes.CDLL(path)
     for name, (res, args) in SIGNATURES.items():
         f = getattr(L, name)
