@@ -213,6 +213,44 @@ def main() -> int:
     log(f"apply f64 {ms_apply64:.4f} ms = {gbps64:.0f} GB/s; f32 {ms_apply32:.4f} ms = {gbps32:.0f} GB/s; "
         f"adjoint f64 {ms_adjoint64:.4f} ms = {gbpst64:.0f} GB/s")
 
+    # ---- rooflines of the step's kernels (times: HIP events on the build's
+    # stream around each kernel, medians over 5 timed builds; the apply from
+    # the event-timed repetitions above).  The first is the longest.
+    phase = {k: [] for k in ("ms_count", "ms_emit", "ms_reduce")}
+    for _ in range(5):
+        hb = build(timings=True)
+        sb = hb.stats()
+        for k in phase:
+            phase[k].append(sb[k])
+        hb.free()
+    ms_walk, ms_scatter, ms_reduce = (float(np.median(phase[k])) for k in ("ms_count", "ms_emit", "ms_reduce"))
+    T = st["n_triples"]
+    nchunks = st["n_chunks"]
+    walk_flop = 64 * st["n_subsegments"]
+    scatter_bytes = 16 * T + 16 * T + 12 * nchunks + 8 * 4 * n_in
+    reduce_bytes = 16 * T + 8 * (4 * n_in + 1) + 8 * n_in + 16 * nnz + 4 * n_in
+    rooflines = [
+        {"kernel": "k_apply_planned<double,8> (CSR gather apply, FP64, 16 images)", "bound": "hbm",
+         "achieved": round(gbps64, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+         "frac": round(gbps64 / HBM_PEAK_GBPS, 4), "traffic": None,
+         "bytes_per_launch": by64, "ms_per_launch": round(ms_apply64, 5)},
+        {"kernel": "k_scatter (staged triples -> input-cell buckets)", "bound": "hbm",
+         "achieved": round(scatter_bytes / (ms_scatter * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBPS,
+         "unit": "GB/s", "frac": round(scatter_bytes / (ms_scatter * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+         "traffic": None, "bytes_per_launch": scatter_bytes, "ms_per_launch": round(ms_scatter, 5)},
+        {"kernel": "k_walk_count (edge-sweep counting walk)", "bound": "fp64",
+         "achieved": round(walk_flop / (ms_walk * 1e-3) / 1e12, 3), "peak": FP64_PEAK_TFLOPS,
+         "unit": "TFLOP/s", "frac": round(walk_flop / (ms_walk * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),
+         "traffic": None, "flop_per_launch": walk_flop, "ms_per_launch": round(ms_walk, 5)},
+        {"kernel": "k_reduce_groups (bucket sort + reduceat in LDS)", "bound": "hbm",
+         "achieved": round(reduce_bytes / (ms_reduce * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBPS,
+         "unit": "GB/s", "frac": round(reduce_bytes / (ms_reduce * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+         "traffic": None, "bytes_per_launch": reduce_bytes, "ms_per_launch": round(ms_reduce, 5)},
+    ]
+    rooflines.sort(key=lambda r: -r["ms_per_launch"])
+    log("rooflines: " + "; ".join(f"{r['kernel'].split(' ')[0]} {r['ms_per_launch']:.3f} ms "
+                                   f"{r['achieved']} {r['unit']} ({100 * r['frac']:.1f}%)" for r in rooflines))
+
     result = {
         "metric": METRIC,
         "value": round(1e3 / ms_per_step, 4),
@@ -236,17 +274,8 @@ def main() -> int:
             "edges_per_lane": tune["edges_per_lane"] or 16,
             "block": tune["block"] or 128,
         },
-        "roofline": {
-            "kernel": "k_apply<double,16> (CSR gather apply, FP64, 16 images)",
-            "bound": "hbm",
-            "achieved": round(gbps64, 1),
-            "peak": HBM_PEAK_GBPS,
-            "unit": "GB/s",
-            "frac": round(gbps64 / HBM_PEAK_GBPS, 4),
-            "traffic": None,
-            "bytes_per_launch": by64,
-            "ms_per_launch": round(ms_apply64, 5),
-        },
+        "roofline": rooflines[0],
+        "rooflines": rooflines[1:],
         "build_ms": round(build_ms, 4),
         **({"build_autotune": tuned} if tuned else {}),
         "build_phases_ms": {k[3:]: round(v, 4) for k, v in st.items() if k.startswith("ms_")},

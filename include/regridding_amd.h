@@ -58,6 +58,11 @@ typedef struct rg_build_stats {
     int64_t n_search_anomalies; /* row starts inside the boundary polygon
                                    that the cell search did not place */
     int64_t n_restaged;     /* chunks whose triples overflowed the staging */
+    /* with RG_TUNE_TIMINGS, stream times (HIP events) of: prepare (areas,
+     * static-grid tables), start (row and chunk start states), count (the
+     * counting walk kernel), fixup (fix-up walk, scans and the host wait
+     * for the totals), emit (the scatter kernel), reduce (the bucket
+     * reduction kernels), transpose (both CSR layouts), total */
     double ms_prepare, ms_start, ms_count, ms_fixup, ms_emit, ms_reduce,
         ms_transpose, ms_total;
 } rg_build_stats;

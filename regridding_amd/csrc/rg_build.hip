@@ -1309,15 +1309,16 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
     uint32_t* cursor = (uint32_t*)hist.p;  // reused: one word per bucket
     k_cursor_init<<<grid_for(4 * cell_tot, 256), 256, 0, s>>>(bucket_off.p, cursor,
                                                               4 * cell_tot);
+    tm.mark();  // 5
     k_scatter<<<grid_for(thr_tot, block), block, 0, s>>>(
         d_builds, pair_of_wave.p, thr_tot, start.p, count.p, stage.p, cap,
         chunk_off.p, cursor, rec.p, (uint32_t)T, counters.p + 2, counters.p + 4);
     RG_HIP(hipGetLastError());
+    tm.mark();  // 6
     k_cursor_check<<<grid_for(4 * cell_tot, 256), 256, 0, s>>>(
         bucket_off.p, cursor, 4 * cell_tot, counters.p + 2);
     RG_HIP(hipGetLastError());
     stage.reset();
-    tm.mark();  // 5
 
     // ---- reduce -> CSR by input
     DevBuf<int32_t> nnz_row, long_list, group_mode;
@@ -1327,6 +1328,7 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
     RG_TRY(long_list.alloc(cell_tot, s));
     RG_TRY(group_mode.alloc(n_groups, s));
     RG_TRY(row_off.alloc(cell_tot + 1, s));
+    tm.mark();  // 7
     k_reduce_groups<<<(unsigned)((n_groups + RED_WAVES - 1) / RED_WAVES),
                       RED_WAVES * 64, 0, s>>>(bucket_off.p, rec.p, vol.p, wi_all.p,
                                               cell_tot, nnz_row.p, group_mode.p,
@@ -1334,6 +1336,7 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
     k_bucket_long<<<4, 64, 0, s>>>(bucket_off.p, rec.p, long_list.p,
                                    counters.p + 3, nnz_row.p);
     RG_HIP(hipGetLastError());
+    tm.mark();  // 8
     RG_TRY(scan_i32_to_i64(nnz_row.p, row_off.p, cell_tot, s));
     // per-pair first entry and first triple (slot npairs: totals)
     DevBuf<int64_t> bases;
@@ -1371,7 +1374,6 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
         d_pcs, pair_of_ing.p, n_groups, bucket_off.p, row_off.p, group_mode.p,
         rec.p, rp_in.p, c_in.p, v_in.p, hist_out.p);
     RG_HIP(hipGetLastError());
-    tm.mark();  // 6
     RG_TRY(scan_i32_to_i64(hist_out.p, off_out.p, out_tot, s));
     RG_TRY(fill_zero(hist_out.p, out_tot, s));  // cursor
     k_transpose_scatter<<<grid_for(cell_tot, 128), 128, 0, s>>>(
@@ -1382,7 +1384,7 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
                   SORT_WAVES * 64, 0, s>>>(d_pcs, pair_of_outg.p, off_out.p, out_tot,
                                            c_out.p, v_out.p, rp_out.p);
     RG_HIP(hipGetLastError());
-    tm.mark();  // 7
+    tm.mark();  // 9
 
     // ---- handles
     std::vector<rg_weights*> made;
@@ -1405,10 +1407,10 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
         batch.ms_start = tm.ms(1, 2);
         batch.ms_count = tm.ms(2, 3);
         batch.ms_fixup = tm.ms(3, 4);
-        batch.ms_emit = tm.ms(4, 5);
-        batch.ms_reduce = tm.ms(5, 6);
-        batch.ms_transpose = tm.ms(6, 7);
-        batch.ms_total = tm.ms(0, 7);
+        batch.ms_emit = tm.ms(5, 6);
+        batch.ms_reduce = tm.ms(7, 8);
+        batch.ms_transpose = tm.ms(8, 9);
+        batch.ms_total = tm.ms(0, 9);
     }
     for (int64_t k = 0; k < npairs; k++) {
         const rg_pair_desc& d = pairs[k];
