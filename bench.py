@@ -247,6 +247,16 @@ def main() -> int:
          "unit": "GB/s", "frac": round(reduce_bytes / (ms_reduce * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
          "traffic": None, "bytes_per_launch": reduce_bytes, "ms_per_launch": round(ms_reduce, 5)},
     ]
+    # measured HBM bytes per launch (two rocprofv3 counter passes over this
+    # workload, committed under profiles/), when they describe this workload
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic_plain_large.json")
+    if not args.small and os.path.exists(tpath):
+        with open(tpath) as f:
+            measured = json.load(f).get("kernels", {})
+        for r in rooflines:
+            m = measured.get(r["kernel"].split(" ")[0])
+            if m:
+                r["traffic"] = m["read_bytes"] + m["write_bytes"]
     rooflines.sort(key=lambda r: -r["ms_per_launch"])
     log("rooflines: " + "; ".join(f"{r['kernel'].split(' ')[0]} {r['ms_per_launch']:.3f} ms "
                                    f"{r['achieved']} {r['unit']} ({100 * r['frac']:.1f}%)" for r in rooflines))
