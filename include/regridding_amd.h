@@ -248,6 +248,43 @@ int rg_find_indices_1d(const double* x_input, int64_t m,
                        int method, int64_t fill_value, int64_t* result,
                        void* stream);
 
+/* Replaces weights_conservative_1d (_weights/_weights_conservative_1d/
+ * _weights_conservative_1d.py:12-205) followed by _coalesce
+ * (_weights_arrays.py:44) for D elements, in two calls.  x_input is
+ * (D, n_input) vertices, x_output (D, n_output).  The count call fills
+ * counts (D*(n_input-1) int32: triples per input cell) and offsets
+ * (D*(n_input-1)+1 int64, their exclusive scan) and returns the total; the
+ * emit call writes the triples (the reference's (possibly negative) indices)
+ * in coalesced order: element by element, sorted by (input, output).
+ * weights_input is (D, n_input-1) or NULL. */
+int rg_conservative_1d_count(const double* x_input, int64_t n_input,
+                             const double* x_output, int64_t n_output, int64_t D,
+                             int32_t* counts, int64_t* offsets, int64_t* total,
+                             void* stream);
+int rg_conservative_1d_emit(const double* x_input, int64_t n_input,
+                            const double* x_output, int64_t n_output, int64_t D,
+                            const double* weights_input, const int32_t* counts,
+                            const int64_t* offsets, int64_t* indices_input,
+                            int64_t* indices_output, double* values, void* stream);
+
+/* Replaces _weights_transposed.py:309-325 _cell_volume_1d (cell_length of
+ * _weights_conservative_1d/_grids.py:11-37): the signed length of every cell
+ * of D elements of n vertices; length is (D, n-1). */
+int rg_cell_length_1d(const double* x, int64_t n, int64_t D, double* length,
+                      void* stream);
+
+/* Replaces _weights_multilinear.py:148-206 (_weights_from_indices_multilinear_1d
+ * with the clamping of :95-118) for D elements: from the searchsorted indices
+ * of the output vertices (D, n_output), writes 2*n_output triples per element
+ * in the reference's storage order; bounds_nan = 1 marks outside points NaN.
+ * n_outside receives the number of outside points (for bounds="raise").
+ * weights_input is (D, n_input) or NULL. */
+int rg_multilinear_1d(const double* x_input, int64_t n_input, const double* x_output,
+                      int64_t n_output, int64_t D, const int64_t* indices,
+                      const double* weights_input, int32_t bounds_nan,
+                      int64_t* indices_input, int64_t* indices_output, double* values,
+                      int64_t* n_outside, void* stream);
+
 /* Test hook (no reference counterpart): make the next call that reaches the
  * named site ("build", "apply", "alloc") fail with RG_ERR_INJECTED.  NULL
  * clears pending injections. */

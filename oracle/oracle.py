@@ -231,3 +231,122 @@ def find_indices_1d(x_input, x_output, fill_value: int, method: str):
     }[method]
     getattr(lib(), name)(_d(a), m, _d(v), n, D, int(fill_value), _i(res))
     return res
+
+
+# ---------------------------------------------------------------------------
+# 1-D weights (pure Python restatements: the sweeps are O(n + m) per element)
+# ---------------------------------------------------------------------------
+
+_MAXSIZE = np.iinfo(np.int64).max  # sys.maxsize
+
+
+def _index_of_point_1d(point: float, grid: np.ndarray) -> int:
+    """_weights_conservative_1d/_grids.py:40-73: bisection, right index."""
+    lo, hi = 0, grid.shape[0]
+    while hi - lo > 1:
+        mid = (lo + hi) // 2
+        if grid[mid] > point:
+            hi = mid
+        else:
+            lo = mid
+    return hi
+
+
+def conservative_1d(x_input, x_output, weights_input=None):
+    """
+    One element of the 1-D conservative sweep, statement by statement after
+    ``_weights_conservative_1d.py:57-318``: the uncoalesced triples in the
+    reference's emission order, with its negative (wraparound) indices for
+    descending grids.
+    """
+    x_sweep = np.asarray(x_input, dtype=np.float64)
+    x_static = np.asarray(x_output, dtype=np.float64)
+    num_sweep = x_sweep.shape[0]
+    num_static = x_static.shape[0]
+    reversed_sweep = not (x_sweep[0] < x_sweep[-1])
+    if reversed_sweep:
+        x_sweep = x_sweep[::-1]
+    reversed_static = not (x_static[0] < x_static[-1])
+    if reversed_static:
+        x_static = x_static[::-1]
+    x_static_left, x_static_right = x_static[0], x_static[-1]
+    length_input = x_sweep[1:] - x_sweep[:-1]
+    out = []
+    index_sweep = 0
+    point_1 = x_sweep[0]
+    if x_static_left == point_1:
+        outside, index_static = False, 0
+    elif x_static_left < point_1 < x_static_right:
+        outside, index_static = False, _index_of_point_1d(point_1, x_static) - 1
+    else:
+        outside, index_static = True, _MAXSIZE
+    while index_sweep < num_sweep - 1:
+        point_2 = x_sweep[index_sweep + 1]
+        if outside:  # _step_outside_static, :193-240
+            point_static = x_static[0]
+            if point_1 < point_static < point_2:
+                index_static = 0
+                point_2 = point_static
+            elif point_static == point_2:
+                index_sweep += 1
+                index_static = 0
+            else:
+                index_sweep += 1
+            if index_static < _MAXSIZE:
+                outside = False
+        else:  # _step_inside_static, :243-318
+            index_input = ~index_sweep if reversed_sweep else index_sweep
+            index_output = ~index_static if reversed_static else index_static
+            point_static = x_static[index_static + 1]
+            if point_1 < point_static < point_2:
+                index_static += 1
+                point_2 = point_static
+            elif point_static == point_2:
+                index_static += 1
+                index_sweep += 1
+            else:
+                index_sweep += 1
+            length = point_2 - point_1
+            ratio = length / length_input[index_input]
+            if weights_input is not None:
+                ratio = ratio * weights_input[index_input]
+            out.append((index_input, index_output, ratio))
+            if not (0 <= index_static < num_static - 1):
+                break
+        point_1 = point_2
+    if not out:
+        return np.empty(0, np.int64), np.empty(0, np.int64), np.empty(0)
+    ii, io, w = zip(*out)
+    return np.array(ii, np.int64), np.array(io, np.int64), np.array(w, np.float64)
+
+
+def multilinear_1d(indices, x_input, x_output, weights_input=None, bounds_nan=False):
+    """
+    One element of ``_weights_multilinear.py:95-118`` (index clamping) and
+    ``:148-206`` (two weights per output vertex, storage order 2i, 2i+1),
+    from the searchsorted indices of the output vertices.
+    """
+    x_input = np.asarray(x_input, dtype=np.float64)
+    x_output = np.asarray(x_output, dtype=np.float64)
+    index = np.asarray(indices, dtype=np.int64)
+    index_max = x_input.shape[0] - 2
+    outside = (index < 0) | (index > index_max)
+    below = x_output < x_input[:1]
+    i0 = np.where(below, 0, np.clip(index, 0, index_max))
+    n = x_output.shape[0]
+    ii = np.empty(2 * n, np.int64)
+    io = np.empty(2 * n, np.int64)
+    w = np.empty(2 * n)
+    for i in range(n):
+        a = int(i0[i])
+        x0, x1, x = x_input[a], x_input[a + 1], x_output[i]
+        w1 = (x - x0) / (x1 - x0)
+        w0 = 1 - w1
+        if weights_input is not None:
+            w0 = w0 * weights_input[a]
+            w1 = w1 * weights_input[a + 1]
+        if bounds_nan and outside[i]:
+            w0 = w1 = np.nan
+        ii[2 * i], io[2 * i], w[2 * i] = a, i, w0
+        ii[2 * i + 1], io[2 * i + 1], w[2 * i + 1] = a + 1, i, w1
+    return ii, io, w, int(outside.sum())

@@ -807,8 +807,12 @@ void rgo_apply(int64_t nnz, const int64_t *in, const int64_t *out,
         const double *x = vin + d * n_in;
         double *y = vout + d * n_out;
         for (int64_t q = 0; q < nnz; q++) {
-            double p = w[q] * x[in[q]];
-            y[out[q]] += p;
+            /* numba indexing wraps negative indices (1-D builders emit them
+             * for descending grids) */
+            const int64_t a = in[q] < 0 ? in[q] + n_in : in[q];
+            const int64_t b = out[q] < 0 ? out[q] + n_out : out[q];
+            double p = w[q] * x[a];
+            y[b] += p;
         }
     }
 }

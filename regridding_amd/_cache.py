@@ -10,6 +10,8 @@ very arrays (or on their ``transpose_weights`` swap) applies the resident
 CSR instead of uploading and regrouping the triples again.
 
 Entries are keyed by the identity of the three arrays and die with them.
+A triple may also be registered as part ``d`` of ``D`` of a block-diagonal
+handle (the 1-D builders put all elements of one call in one handle).
 The registered arrays are made read-only, so a cached handle can never
 disagree with the host arrays it stands for; arrays the registry has not
 seen (unpickled, hand-made, modified copies) go through the import path.
@@ -37,30 +39,41 @@ def _drop(key: int) -> None:
 
 
 def register(indices_input: np.ndarray, indices_output: np.ndarray, values: np.ndarray,
-             handle: Weights) -> None:
+             handle: Weights, part: tuple[int, int, int, int] | None = None) -> None:
+    """``part``: ``(d, D, n_in, n_out)`` when ``handle`` is block-diagonal
+    and this triple is its block d (n_in x n_out cells per block)."""
     for a in (indices_input, indices_output, values):
         a.flags.writeable = False
     key = id(values)
     with _lock:
-        _entries[key] = (weakref.ref(indices_input), weakref.ref(indices_output), handle)
+        _entries[key] = (weakref.ref(indices_input), weakref.ref(indices_output), handle, part)
     weakref.finalize(values, _drop, key)
 
 
 def lookup(a: np.ndarray, b: np.ndarray, values: np.ndarray) -> tuple[Weights, bool] | None:
     """
-    ``(handle, transposed)`` if ``(a, b, values)`` is a registered triple
-    (``transposed=False``) or its index swap (``transposed=True``).
+    ``(handle, transposed)`` if ``(a, b, values)`` is a registered whole
+    triple (``transposed=False``) or its index swap (``transposed=True``).
     """
+    hit = lookup_part(a, b, values)
+    if hit is None or hit[2] is not None:
+        return None
+    return hit[0], hit[1]
+
+
+def lookup_part(a: np.ndarray, b: np.ndarray, values: np.ndarray):
+    """``(handle, transposed, part)`` for a registered triple or its swap
+    (``part`` None for a whole handle), else None."""
     with _lock:
         e = _entries.get(id(values))
     if e is None:
         return None
-    ri, ro, handle = e
+    ri, ro, handle, part = e
     ii, io = ri(), ro()
     if ii is None or io is None:
         return None
     if a is ii and b is io:
-        return handle, False
+        return handle, False, part
     if a is io and b is ii:
-        return handle, True
+        return handle, True, part
     return None

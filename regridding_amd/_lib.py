@@ -181,6 +181,12 @@ SIGNATURES = {
     "rg_serve_wait": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int64)]),
     "rg_serve_stats_get": (ctypes.c_int, [_vp, ctypes.POINTER(rg_serve_stats)]),
     "rg_serve_destroy": (ctypes.c_int, [_vp]),
+    "rg_conservative_1d_count": (ctypes.c_int, [_vp, _i64, _vp, _i64, _i64, _vp, _vp, _pi64, _vp]),
+    "rg_conservative_1d_emit": (ctypes.c_int, [_vp, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp,
+                                               _vp, _vp, _vp]),
+    "rg_cell_length_1d": (ctypes.c_int, [_vp, _i64, _i64, _vp, _vp]),
+    "rg_multilinear_1d": (ctypes.c_int, [_vp, _i64, _vp, _i64, _i64, _vp, _vp, ctypes.c_int32,
+                                         _vp, _vp, _vp, _pi64, _vp]),
     "rg_weights_info": (ctypes.c_int, [_vp, _pi64, _pi64, _pi64]),
     "rg_weights_stats": (ctypes.c_int, [_vp, ctypes.POINTER(rg_build_stats)]),
     "rg_weights_export": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),

@@ -25,9 +25,15 @@ __all__ = ["regrid_from_weights", "regrid", "import_weights", "prepare_apply", "
 
 
 def import_weights(indices_input, indices_output, values, n_in: int, n_out: int) -> Weights:
-    """Upload arbitrary triples (any order) as a device handle."""
+    """Upload arbitrary triples (any order) as a device handle.  Negative
+    indices wrap around, as the reference's numba indexing does (the 1-D
+    builders emit them for descending grids)."""
     ii = np.ascontiguousarray(indices_input, dtype=np.int64)
     io = np.ascontiguousarray(indices_output, dtype=np.int64)
+    if ii.size and ii.min() < 0:
+        ii = np.where(ii < 0, ii + int(n_in), ii)
+    if io.size and io.min() < 0:
+        io = np.where(io < 0, io + int(n_out), io)
     w = np.ascontiguousarray(np.asarray(values), dtype=np.float64)
     if not (ii.shape == io.shape == w.shape and ii.ndim == 1):
         raise ValueError("weight arrays must be 1-D and of one length")
@@ -160,6 +166,33 @@ def _handle_for(element, n_in: int, n_out: int) -> tuple[Weights, bool]:
     return import_weights(ii, io, w, n_in, n_out), False
 
 
+def _block_diagonal(flat, n_in: int, n_out: int):
+    """``(handle, transposed)`` when element d of ``flat`` is block d of one
+    block-diagonal handle for every d (the 1-D builders' output applied as
+    returned), else None."""
+    D = flat.size
+    if D == 0:
+        return None
+    first = None
+    for d in range(D):
+        ii, io, w = flat[d]
+        if not (isinstance(ii, np.ndarray) and isinstance(io, np.ndarray)):
+            return None
+        hit = _cache.lookup_part(ii, io, np.asarray(w))
+        if hit is None or hit[2] is None:
+            return None
+        handle, transposed, (k, nparts, bi, bo) = hit
+        if transposed:
+            bi, bo = bo, bi
+        if k != d or nparts != D or (bi, bo) != (n_in, n_out):
+            return None
+        if first is None:
+            first = (handle, transposed)
+        elif handle is not first[0] or transposed != first[1]:
+            return None
+    return first
+
+
 def apply_device(handle: Weights, transposed: bool, x: np.ndarray, n_out: int) -> np.ndarray:
     """(D, n_in) host float64 -> (D, n_out) host float64 through the GPU."""
     D = x.shape[0]
@@ -202,6 +235,12 @@ def regrid_from_weights(
     x = prep["values_input"]
     n_in, n_out = prep["n_in"], prep["n_out"]
     D = x.shape[0]
+    whole = _block_diagonal(flat, n_in, n_out)
+    if whole is not None:
+        # every element is its own block of one handle, in order: one launch
+        handle, transposed = whole
+        y = apply_device(handle, transposed, x.reshape(1, D * n_in), D * n_out)
+        return finish_apply(prep, y.reshape(D, n_out))
     result = np.zeros((D, n_out))
     # group the elements by weight object: broadcast weights are one object
     groups: dict[int, list[int]] = {}

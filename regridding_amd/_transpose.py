@@ -59,7 +59,14 @@ def _cell_volume(grid: tuple[np.ndarray, ...], axis: tuple[int, ...]) -> np.ndar
         result = np.reshape(result, x.shape[:-2] + result.shape[-2:])
         return np.moveaxis(result, axis_last, axis)
     if len(grid) == 1:
-        raise NotImplementedError("1-D conservative transposes are not implemented in regridding_amd yet")
+        (x,) = grid
+        x = np.ascontiguousarray(np.moveaxis(x, axis, axis_last).reshape((-1,) + shape_last),
+                                 dtype=np.float64)
+        D, n = x.shape
+        out = DeviceArray.empty((D, n - 1), np.float64)
+        check(lib().rg_cell_length_1d(DeviceArray.from_host(x).ptr, n, D, out.ptr, 0))
+        result = out.to_host()
+        return np.moveaxis(result, axis_last, axis)
     raise ValueError("Grids greater than 2D not supported.")
 
 

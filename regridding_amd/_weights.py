@@ -8,13 +8,13 @@ from typing import Literal, Sequence
 
 import numpy as np
 
-from . import _cache, _util
-from ._conservative import build_2d
+from . import _cache, _line, _util
+from ._conservative import build_2d_batched
 
-__all__ = ["weights", "conservative_handles"]
+__all__ = ["weights", "conservative_weights"]
 
 
-def conservative_handles(
+def conservative_weights(
     coordinates_input,
     coordinates_output,
     axis_input=None,
@@ -24,11 +24,12 @@ def conservative_handles(
     seed=_util.SEED_DEFAULT,
 ):
     """
-    Normalise like ``_weights_conservative`` (``_weights_conservative.py:
-    11-146``) and build every orthogonal element on the device.
-
-    Returns ``(handles, shape_values_input, shape_values_output)`` with
-    ``handles`` an object array of :class:`regridding_amd._lib.Weights`.
+    ``_weights_conservative`` (``_weights_conservative.py:11-146``) followed
+    by ``_weights_to_arrays`` (``_weights_arrays.py:8-41``): normalise, then
+    build every orthogonal element on the device -- 2-D elements in one
+    batched build, 1-D elements in one block-diagonal sweep -- and return
+    the coalesced per-element triples (registered with their device handle
+    for :func:`regrid_from_weights`).
     """
     perturb = _util.resolve_perturb(perturb, coordinates_input)
     (
@@ -62,25 +63,39 @@ def conservative_handles(
         weights_input = np.broadcast_to(weights_input, shape_values_input)
 
     if len(axis_input) == 1:
-        raise NotImplementedError(
-            "1-D conservative weights are not implemented in regridding_amd yet"
-        )
+        (x_input,) = coordinates_input
+        (x_output,) = coordinates_output
+        x_input = np.moveaxis(x_input, axis_input, ~0)
+        x_output = np.moveaxis(x_output, axis_output, ~0)
+        x_input = x_input.reshape(-1, x_input.shape[~0])
+        x_output = x_output.reshape(-1, x_output.shape[~0])
+        if weights_input is not None:
+            weights_input = np.moveaxis(weights_input, axis_input, ~0)
+            weights_input = weights_input.reshape(-1, weights_input.shape[~0])
+        result = _line.conservative_1d(x_input, x_output, weights_input, shape_orthogonal)
+        return result, shape_values_input, shape_values_output
     if len(axis_input) != 2:
         raise NotImplementedError(
             "Regridding operations greater than 2D are not supported"
         )
 
-    handles = np.empty(shape_orthogonal, dtype=object)
     x_in, y_in = coordinates_input
     x_out, y_out = coordinates_output
+    items = []
     for index in np.ndindex(*shape_orthogonal):
         sl_in = _util.orthogonal_slice(index, axis_input)
         sl_out = _util.orthogonal_slice(index, axis_output)
         # the reference indexes weights_input with the bare orthogonal index
         # (_weights_conservative.py:126)
         wi = weights_input[index] if weights_input is not None else None
-        handles[index] = build_2d((x_in[sl_in], y_in[sl_in]), (x_out[sl_out], y_out[sl_out]), wi)
-    return handles, shape_values_input, shape_values_output
+        items.append(((x_in[sl_in], y_in[sl_in]), (x_out[sl_out], y_out[sl_out]), wi))
+    handles = build_2d_batched(items)
+    result = np.empty(len(handles), dtype=object)
+    for k, h in enumerate(handles):
+        ii, io, w = h.export()
+        _cache.register(ii, io, w, h)
+        result[k] = (ii, io, w)
+    return result.reshape(shape_orthogonal), shape_values_input, shape_values_output
 
 
 def weights(
@@ -101,21 +116,27 @@ def weights(
     (``regridding/_weights/_weights.py:13-196``): an object array over the
     orthogonal axes whose elements are ``(indices_input, indices_output,
     values)`` flat arrays sorted by ``(input, output)`` with unique pairs,
-    plus the input and output value shapes.  The conservative 2-D build runs
-    on the GPU; the returned arrays stay resident there for
-    :func:`regrid_from_weights` (they are returned read-only for that
-    reason).
+    plus the input and output value shapes.  The builds run on the GPU; the
+    returned arrays stay resident there for :func:`regrid_from_weights`
+    (they are returned read-only for that reason).
     """
     unit_weights = getattr(weights_input, "unit", None)
     if unit_weights is not None:
         weights_input = getattr(weights_input, "value")
 
     if method == "multilinear":
-        raise NotImplementedError(
-            "method='multilinear' is not implemented in regridding_amd yet"
+        result, shape_values_input, shape_values_output = _line.multilinear_weights(
+            coordinates_input=coordinates_input,
+            coordinates_output=coordinates_output,
+            axis_input=axis_input,
+            axis_output=axis_output,
+            weights_input=weights_input,
+            bounds=bounds,
+            perturb=perturb,
+            seed=seed,
         )
     elif method == "conservative":
-        handles, shape_values_input, shape_values_output = conservative_handles(
+        result, shape_values_input, shape_values_output = conservative_weights(
             coordinates_input=coordinates_input,
             coordinates_output=coordinates_output,
             axis_input=axis_input,
@@ -127,14 +148,9 @@ def weights(
     else:
         raise ValueError(f"unrecognized method '{method}'")
 
-    result = np.empty(handles.shape, dtype=object)
-    flat_h = handles.reshape(-1)
-    flat_r = result.reshape(-1)
-    for k in range(flat_h.size):
-        h = flat_h[k]
-        ii, io, w = h.export()
-        _cache.register(ii, io, w, h)
-        if unit_weights is not None:
-            w = w << unit_weights
-        flat_r[k] = (ii, io, w)
+    if unit_weights is not None:
+        flat = result.reshape(-1)
+        for k in range(flat.size):
+            ii, io, w = flat[k]
+            flat[k] = (ii, io, w << unit_weights)
     return result, shape_values_input, shape_values_output

@@ -63,3 +63,77 @@ def regrid(coordinates_input, coordinates_output, values_input, values_output=No
                 perturb=perturb, seed=seed)
     return regrid_from_weights(*w, values_input=values_input, values_output=values_output,
                                axis_input=axis_input, axis_output=axis_output)
+
+
+def weights_conservative_1d(coordinates_input, coordinates_output, axis_input=None,
+                            axis_output=None, weights_input=None, perturb=None,
+                            seed=_util.SEED_DEFAULT):
+    """Conservative 1-D weights, normalised as ``_weights_conservative``'s
+    1-D branch, swept and coalesced by the oracle."""
+    perturb = _util.resolve_perturb(perturb, coordinates_input)
+    (ci, co, axis_input, axis_output, shape_in, shape_out, shape_orth) = (
+        _util.normalize_coordinates(coordinates_input, coordinates_output, axis_input,
+                                    axis_output, perturb, seed)
+    )
+    sv_in = list(shape_in)
+    for ax in axis_input:
+        sv_in[ax] -= 1
+    sv_out = list(shape_out)
+    for ax in axis_output:
+        sv_out[ax] -= 1
+    assert len(axis_input) == 1
+    x_in = np.moveaxis(ci[0], axis_input, ~0)
+    x_out = np.moveaxis(co[0], axis_output, ~0)
+    x_in = x_in.reshape(-1, x_in.shape[~0])
+    x_out = x_out.reshape(-1, x_out.shape[~0])
+    wi = None
+    if weights_input is not None:
+        wi = np.broadcast_to(weights_input, tuple(sv_in))
+        wi = np.moveaxis(wi, axis_input, ~0)
+        wi = wi.reshape(-1, wi.shape[~0])
+    flat = np.empty(x_in.shape[0], dtype=object)
+    for d in range(x_in.shape[0]):
+        trip = oracle.conservative_1d(x_in[d], x_out[d], None if wi is None else wi[d])
+        flat[d] = oracle.coalesce(*trip)
+    return flat.reshape(shape_orth), tuple(sv_in), tuple(sv_out)
+
+
+def weights_multilinear(coordinates_input, coordinates_output, axis_input=None,
+                        axis_output=None, weights_input=None, bounds="extrapolate",
+                        perturb=None, seed=_util.SEED_DEFAULT):
+    """Multilinear 1-D weights as ``_weights_multilinear``: searchsorted
+    indices from the oracle, clamping and weights from the oracle."""
+    if isinstance(coordinates_input, np.ndarray):
+        coordinates_input = (coordinates_input,)
+    if isinstance(coordinates_output, np.ndarray):
+        coordinates_output = (coordinates_output,)
+    (ci0, co0, ai0, ao0, *_rest) = _util.normalize_coordinates(
+        coordinates_input, coordinates_output, axis_input, axis_output, False, seed)
+    # find_indices(method="searchsorted") on the unperturbed coordinates
+    x_in0 = np.moveaxis(ci0[0], ai0, ~0)
+    x_out0 = np.moveaxis(co0[0], ao0, ~0)
+    shp = x_out0.shape
+    idx = oracle.find_indices_1d(x_in0.reshape(-1, x_in0.shape[~0]),
+                                 x_out0.reshape(-1, shp[~0]), np.iinfo(int).max,
+                                 "searchsorted").reshape(shp)
+    perturb = False if perturb is None else perturb
+    (ci, co, axis_input, axis_output, shape_in, shape_out, shape_orth) = (
+        _util.normalize_coordinates(coordinates_input, coordinates_output, axis_input,
+                                    axis_output, perturb, seed)
+    )
+    x_in = np.moveaxis(ci[0], axis_input, ~0).reshape(-1, shape_in[axis_input[0]])
+    x_out = np.moveaxis(co[0], axis_output, ~0).reshape(-1, shape_out[axis_output[0]])
+    idx = idx.reshape(-1, shape_out[axis_output[0]])
+    wi = None
+    if weights_input is not None:
+        wi = np.broadcast_to(weights_input, shape_in)
+        wi = np.moveaxis(wi, axis_input, ~0).reshape(-1, shape_in[axis_input[0]])
+    flat = np.empty(x_in.shape[0], dtype=object)
+    for d in range(x_in.shape[0]):
+        ii, io, w, n_out = oracle.multilinear_1d(idx[d], x_in[d], x_out[d],
+                                                 None if wi is None else wi[d],
+                                                 bounds == "nan")
+        if bounds == "raise" and n_out:
+            raise ValueError("outside")
+        flat[d] = oracle.coalesce(ii, io, w)
+    return flat.reshape(shape_orth), tuple(shape_in), tuple(shape_out)
