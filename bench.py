@@ -185,20 +185,19 @@ def main() -> int:
 
     sweep = []
     if args.apply_sweep:
-        shapes = [(tw, ipp, kib) for tw in (-16, 64, 32, 16, 8) for ipp in (16, 8, 4)
-                  for kib in (32, 64)]
-        for tw, ipp, kib in shapes:
-            if tw < 0 and kib == 64:
-                continue
-            check(L.rg_set_apply_tuning(tw, ipp, kib))
+        shapes = [(tw, ipp, kib, rounds) for rounds in (0, 1) for tw in (32, 16, 8)
+                  for ipp in (16, 8, 4, 2) for kib in (16, 32, 64)]
+        for tw, ipp, kib, rounds in shapes:
+            check(L.rg_set_apply_tuning(tw, ipp, kib, rounds))
             ms = time_apply(L.rg_apply_f64, dx64, dy64, 0)
             ms_t = time_apply(L.rg_apply_f64, dy64, dt64, 1)
-            sweep.append({"tile_w": tw, "images_per_pass": ipp, "lds_kib": kib,
+            sweep.append({"tile_w": tw, "images_per_pass": ipp, "lds_kib": kib, "rounds": rounds,
                           "ms": round(ms, 5), "adjoint_ms": round(ms_t, 5)})
-            log(f"apply sweep tile_w={tw} images/pass={ipp} lds={kib}KiB: "
+            log(f"apply sweep tile_w={tw} images/pass={ipp} lds={kib}KiB rounds={rounds}: "
                 f"{ms:.4f} ms, adjoint {ms_t:.4f} ms")
         best = min(sweep, key=lambda r: r["ms"])
-        check(L.rg_set_apply_tuning(best["tile_w"], best["images_per_pass"], best["lds_kib"]))
+        check(L.rg_set_apply_tuning(best["tile_w"], best["images_per_pass"], best["lds_kib"],
+                                    best["rounds"]))
         log(f"apply sweep best: {best}")
 
     ms_apply64 = time_apply(L.rg_apply_f64, dx64, dy64, 0)
@@ -230,7 +229,7 @@ def main() -> int:
     scatter_bytes = 16 * T + 16 * T + 12 * nchunks + 8 * 4 * n_in
     reduce_bytes = 16 * T + 8 * (4 * n_in + 1) + 8 * n_in + 16 * nnz + 4 * n_in
     rooflines = [
-        {"kernel": "k_apply_planned<double,8> (CSR gather apply, FP64, 16 images)", "bound": "hbm",
+        {"kernel": "k_apply_rounds<double,8,8> (CSR gather apply, FP64, 16 images)", "bound": "hbm",
          "achieved": round(gbps64, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
          "frac": round(gbps64 / HBM_PEAK_GBPS, 4), "traffic": None,
          "bytes_per_launch": by64, "ms_per_launch": round(ms_apply64, 5)},

@@ -188,9 +188,12 @@ int rg_weights_set_shape(rg_weights_t w, int64_t ncx_in, int64_t ncy_in,
  * value): tile_w output cells along the last axis per tile row of a
  * 256-cell tile (negative: same width, staged input patches off),
  * images_per_pass images accumulated per pass over the weights (1, 2, 4, 8
- * or 16), and lds_kib the staged patch size per block (KiB, 1..64) above
- * which fewer images are staged per pass.  Has no effect on results. */
-int rg_set_apply_tuning(int tile_w, int images_per_pass, int lds_kib);
+ * or 16), lds_kib the staged patch size per block (KiB, 1..64) above which
+ * fewer images are staged per pass, and rounds (1: one block per tile loops
+ * over the image groups with its CSR entries in registers; 0: one block per
+ * tile and image group; -1 keeps the current choice).  Has no effect on
+ * results. */
+int rg_set_apply_tuning(int tile_w, int images_per_pass, int lds_kib, int rounds);
 
 /* Replaces _regrid/_regrid_from_weights.py:166 _regrid_from_weights for
  * `batch` elements that share one weight matrix: values_input is

@@ -193,7 +193,7 @@ SIGNATURES = {
     "rg_weights_import": (ctypes.c_int, [_vp, _vp, _vp, _i64, _i64, _i64, ctypes.POINTER(_vp), _vp]),
     "rg_weights_free": (ctypes.c_int, [_vp]),
     "rg_weights_set_shape": (ctypes.c_int, [_vp, _i64, _i64, _i64, _i64]),
-    "rg_set_apply_tuning": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "rg_set_apply_tuning": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "rg_apply_f64": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _i64, _vp]),
     "rg_apply_f32": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _i64, _vp]),
     "rg_find_cells_2d": (ctypes.c_int, [_vp, _vp, _i64, _i64, _vp, _vp, _i64, _vp, _vp]),

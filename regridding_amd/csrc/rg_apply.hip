@@ -283,6 +283,117 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_apply_planned(
         if (m < nd) yb[m * nrows] = acc[m];
 }
 
+// The planned apply with all images in one block: the lane's CSR entries
+// (up to MAXE) are loaded once into registers, then for each group of DB
+// images the block stages the input patch in LDS and gathers from it.  The
+// entries and row pointers are read once per launch instead of once per
+// image group, and the blocks (one per tile) stay resident longer.
+template <class T, int DB, int MAXE>
+__global__ __launch_bounds__(APPLY_BLOCK) void k_apply_rounds(
+    const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+    const uint16_t* __restrict__ off, const int32_t* __restrict__ box,
+    const T* __restrict__ w, const T* __restrict__ x, T* __restrict__ y,
+    int64_t nrows, int64_t ncols, int64_t src_gy, int64_t batch, ApplyGrid G,
+    int64_t lds_bytes)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    T* patch = reinterpret_cast<T*>(lds_raw);
+    const int64_t nb = gridDim.x;  // a multiple of NUM_XCD
+    const int64_t b = blockIdx.x;
+    const int64_t tile = (b % NUM_XCD) * (nb / NUM_XCD) + b / NUM_XCD;
+    if (tile >= G.ntiles) return;  // uniform per block
+    const int32_t i0 = box[4 * tile + 0], j0 = box[4 * tile + 1];
+    const int32_t rows = box[4 * tile + 2], cols = box[4 * tile + 3];
+    const int32_t ncell = rows * cols;
+    const int32_t ti = (int32_t)(tile / G.ntx), tj = (int32_t)(tile - (int64_t)ti * G.ntx);
+    const int32_t tw = (int32_t)G.tw;
+    const int64_t i = (int64_t)ti * G.th + (int32_t)threadIdx.x / tw;
+    const int64_t j = (int64_t)tj * G.tw + (int32_t)threadIdx.x % tw;
+    const bool valid = i < G.gx && j < G.gy;
+    const int64_t r = valid ? i * G.gy + j : 0;
+    int32_t s = 0, e = 0;
+    if (valid) {
+        s = rowptr[r];
+        e = rowptr[r + 1];
+    }
+    const int32_t cnt = e - s;
+    int32_t ro[MAXE];
+    T rw[MAXE];
+#pragma unroll
+    for (int q = 0; q < MAXE; q++) {
+        if (q < cnt) {
+            ro[q] = off[s + q];
+            rw[q] = w[s + q];
+        }
+    }
+    const bool fits = ncell > 0 && (int64_t)ncell * DB * (int64_t)sizeof(T) <= lds_bytes;
+    const float inv_cell = 1.0f / (float)(ncell > 0 ? ncell : 1);
+    const float inv_cols = 1.0f / (float)(cols > 0 ? cols : 1);
+    for (int64_t d0 = 0; d0 < batch; d0 += DB) {
+        const int64_t nd = batch - d0 < DB ? batch - d0 : DB;
+        if (fits) {
+            const int32_t total = ncell * (int32_t)nd;
+            const T* xb = x + d0 * ncols + (int64_t)i0 * src_gy + j0;
+            for (int32_t e0 = threadIdx.x; e0 < total; e0 += APPLY_BLOCK * STAGE_UNROLL) {
+                T v[STAGE_UNROLL];
+#pragma unroll
+                for (int u = 0; u < STAGE_UNROLL; u++) {
+                    const int32_t ee = e0 + u * APPLY_BLOCK;
+                    if (ee < total) {
+                        const int32_t m = fast_div(ee, ncell, inv_cell);
+                        const int32_t rem = ee - m * ncell;
+                        const int32_t ii = fast_div(rem, cols, inv_cols);
+                        const int32_t jj = rem - ii * cols;
+                        v[u] = xb[(int64_t)m * ncols + (int64_t)ii * src_gy + jj];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < STAGE_UNROLL; u++) {
+                    const int32_t ee = e0 + u * APPLY_BLOCK;
+                    if (ee < total) patch[ee] = v[u];
+                }
+            }
+            __syncthreads();
+        }
+        if (valid) {
+            T acc[DB];
+#pragma unroll
+            for (int m = 0; m < DB; m++) acc[m] = T(0);
+            if (fits) {
+#pragma unroll
+                for (int q = 0; q < MAXE; q++) {
+                    if (q < cnt) {
+#pragma unroll
+                        for (int m = 0; m < DB; m++)
+                            if (m < nd) acc[m] = acc[m] + rw[q] * patch[m * ncell + ro[q]];
+                    }
+                }
+                for (int32_t q = s + MAXE; q < e; q++) {
+                    const int32_t o = off[q];
+                    const T wq = w[q];
+#pragma unroll
+                    for (int m = 0; m < DB; m++)
+                        if (m < nd) acc[m] = acc[m] + wq * patch[m * ncell + o];
+                }
+            } else {
+                const T* xg = x + d0 * ncols;
+                for (int32_t q = s; q < e; q++) {
+                    const int64_t c = col[q];
+                    const T wq = w[q];
+#pragma unroll
+                    for (int m = 0; m < DB; m++)
+                        if (m < nd) acc[m] = acc[m] + wq * xg[m * ncols + c];
+                }
+            }
+            T* yb = y + d0 * nrows + r;
+#pragma unroll
+            for (int m = 0; m < DB; m++)
+                if (m < nd) yb[m * nrows] = acc[m];
+        }
+        if (fits) __syncthreads();  // the next group overwrites the patch
+    }
+}
+
 __global__ void k_to_f32(const double* __restrict__ a, float* __restrict__ b,
                          int64_t n)
 {
@@ -364,6 +475,7 @@ int g_tile_w = 16;          // tuned on the plain 2049^2 -> 1537^2 pair
 int g_images_per_pass = 8;
 int64_t g_lds_limit = LDS_BYTES;
 bool g_use_plan = true;
+bool g_rounds = true;  // one block per tile looping over image groups
 std::mutex g_plan_mu;
 
 ApplyGrid make_grid(const int64_t* shape, int64_t nrows)
@@ -435,7 +547,24 @@ int launch_apply(rg_weights* W, int dir, const int32_t* rowptr,
     // the direct gather for the largest tiles
     while (P && db > 1 && P->max_cells * db * (int64_t)sizeof(T) > g_lds_limit) db >>= 1;
     dim3 g((unsigned)nb, (unsigned)((batch + db - 1) / db)), bl(APPLY_BLOCK);
-    if (P) {
+    if (P && g_rounds) {
+        int64_t lds = P->max_cells * db * (int64_t)sizeof(T);
+        lds = (lds + 255) / 256 * 256;
+        if (lds > LDS_BYTES) lds = LDS_BYTES;
+        dim3 g1((unsigned)nb);
+#define RG_ROUNDS(D)                                                           \
+    k_apply_rounds<T, D, 8><<<g1, bl, (size_t)lds, s>>>(rowptr, col, P->off, P->box, \
+                                                       w, x, y, nrows, ncols, src_gy, \
+                                                       batch, G, lds)
+        switch (db) {
+        case 16: RG_ROUNDS(16); break;
+        case 8: RG_ROUNDS(8); break;
+        case 4: RG_ROUNDS(4); break;
+        case 2: RG_ROUNDS(2); break;
+        default: RG_ROUNDS(1); break;
+        }
+#undef RG_ROUNDS
+    } else if (P) {
         int64_t lds = P->max_cells * db * (int64_t)sizeof(T);
         lds = (lds + 255) / 256 * 256;
         if (lds > LDS_BYTES) lds = LDS_BYTES;
@@ -631,8 +760,9 @@ int rg_apply_batched_f32(const rg_apply_job* jobs, int64_t njobs, void* stream)
     return launch_jobs_common(jobs, njobs, true, (hipStream_t)stream);
 }
 
-int rg_set_apply_tuning(int tile_w, int images_per_pass, int lds_kib)
+int rg_set_apply_tuning(int tile_w, int images_per_pass, int lds_kib, int rounds)
 {
+    if (rounds >= 0) g_rounds = rounds != 0;
     // negative tile_w: keep the tile width, turn the staged (planned) apply
     // off; the direct-gather kernel computes the same bits
     if (tile_w < 0) {

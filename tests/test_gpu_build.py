@@ -142,7 +142,8 @@ def test_apply_matches_oracle(D):
 
 @pytest.mark.parametrize("tile_w", [-16, 8, 16, 32, 256])
 @pytest.mark.parametrize("ipp,lds_kib", [(1, 32), (4, 1), (16, 64)])
-def test_apply_launch_shapes_same_bits(tile_w, ipp, lds_kib):
+@pytest.mark.parametrize("rounds", [0, 1])
+def test_apply_launch_shapes_same_bits(tile_w, ipp, lds_kib, rounds):
     # planned (LDS-staged) and direct gathers, any tile and image grouping:
     # the per-output accumulation order never changes (the output grid has
     # more than 65536 cells, so the staged path is taken)
@@ -155,11 +156,11 @@ def test_apply_launch_shapes_same_bits(tile_w, ipp, lds_kib):
     ref = oracle.apply(ii, io, w, x, h.n_out)
     ref_t = oracle.apply(io, ii, w, y, h.n_in)
     try:
-        check(lib().rg_set_apply_tuning(tile_w, ipp, lds_kib))
+        check(lib().rg_set_apply_tuning(tile_w, ipp, lds_kib, rounds))
         assert np.array_equal(_apply(h, x), ref)
         assert np.array_equal(_apply(h, y, transpose=True), ref_t)
     finally:
-        check(lib().rg_set_apply_tuning(16, 8, 64))
+        check(lib().rg_set_apply_tuning(16, 8, 64, 1))
 
 
 def test_apply_batched_matches_single():
