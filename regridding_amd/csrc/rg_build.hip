@@ -35,10 +35,11 @@ namespace rg {
 
 // ---- atomics usable from the shared walk code ----------------------------
 
-// Input-cell buckets are split by sweep pass: bucket (cell, pass) at slot
-// 4 * cell + pass, so a cell's four pieces are contiguous (one bucket for
-// the reduction) while the triples one pass writes for neighbouring cells
-// land close together.
+// Input-cell buckets are split by sweep pass and laid out pass-major:
+// bucket (pass p, cell c) at slot p * C + c (C cells in all).  The triples
+// one pass writes for neighbouring cells -- which neighbouring lanes emit at
+// about the same time -- then share cache lines, both for the histogram
+// atomics and for the scattered records.
 
 // Histogram increments are combined in registers: a walk's consecutive
 // triples mostly fall in one or two (cell, pass) buckets (the static cell of
@@ -79,6 +80,7 @@ struct StageSink {
     Triple* __restrict__ stage;
     int32_t* __restrict__ hist;
     int32_t p;
+    int64_t ps;  // bucket (pass p, cell c) is hist[p * ps + c]
     HistCache hc;
     __device__ void emit(int64_t cin, int64_t cout, double a)
     {
@@ -90,7 +92,7 @@ struct StageSink {
             stage[stage_index(g, n, cap)] = t;
         }
         n++;
-        hc.add(hist, 4 * cin + p, 1);
+        hc.add(hist, p * ps + cin, 1);
     }
     __device__ void flush() { hc.flush(hist); }
 };
@@ -98,10 +100,11 @@ struct StageSink {
 struct UncountSink {
     int32_t* hist;
     int32_t p;
+    int64_t ps;
     HistCache hc;
     __device__ void emit(int64_t cin, int64_t, double)
     {
-        hc.add(hist, 4 * cin + p, -1);
+        hc.add(hist, p * ps + cin, -1);
     }
     __device__ void flush() { hc.flush(hist); }
 };
@@ -137,9 +140,10 @@ struct BucketSink {
     uint32_t n_rec;
     int32_t* __restrict__ overflow;
     int32_t p;
+    int64_t ps;
     __device__ void emit(int64_t cin, int64_t cout, double a)
     {
-        put_records(cursor, rec, n_rec, overflow, 4 * cin + p, cout, ordinal, a);
+        put_records(cursor, rec, n_rec, overflow, p * ps + cin, cout, ordinal, a);
         ordinal++;
     }
 };
@@ -470,7 +474,7 @@ __device__ inline void chunk_range(const Build& B, const Pass& P, int64_t c,
 
 __global__ void k_walk_count(const Build* __restrict__ builds,
                              const int32_t* __restrict__ pair_of_wave,
-                             int64_t nthreads_all,
+                             int64_t nthreads_all, int64_t ncells_all,
                              const int32_t* __restrict__ start,
                              int32_t* __restrict__ end,
                              int32_t* __restrict__ count,
@@ -486,7 +490,7 @@ __global__ void k_walk_count(const Build* __restrict__ builds,
     const int64_t g = B.chunk_gbase + r.g;
     int64_t k0, k1;
     chunk_range(B, P, r.c, &k0, &k1);
-    StageSink sink{0, t, cap, stage, hist + 4 * B.cell_gbase, r.p, {}};
+    StageSink sink{0, t, cap, stage, hist + B.cell_gbase, r.p, ncells_all, {}};
     end[g] = walk_chunk(B, P, r.row, k0, k1, start[g], sink);
     sink.flush();
     count[g] = sink.n;
@@ -495,7 +499,8 @@ __global__ void k_walk_count(const Build* __restrict__ builds,
 // One thread per row: make every chunk start where its predecessor ended.
 __global__ void k_fixup(const Build* __restrict__ builds,
                         const int32_t* __restrict__ pair_of_row,
-                        int64_t nrows_all, int32_t* __restrict__ start,
+                        int64_t nrows_all, int64_t ncells_all,
+                        int32_t* __restrict__ start,
                         int32_t* __restrict__ end, int32_t* __restrict__ count,
                         int32_t* __restrict__ hist_all, int32_t* __restrict__ nfix,
                         int32_t* __restrict__ failed,
@@ -509,7 +514,7 @@ __global__ void k_fixup(const Build* __restrict__ builds,
     const Pass& P = B.pass[p];
     const int64_t row = r - B.row_base[p];
     const int64_t g0 = B.chunk_gbase + P.chunk_base + row * P.cpr;
-    int32_t* hist = hist_all + 4 * B.cell_gbase;
+    int32_t* hist = hist_all + B.cell_gbase;
     int32_t fixes = 0;
     for (int64_t c = 1; c < P.cpr; c++) {
         const int64_t g = g0 + c;
@@ -518,11 +523,11 @@ __global__ void k_fixup(const Build* __restrict__ builds,
         if (start[g] == truth) continue;
         int64_t k0, k1;
         chunk_range(B, P, c, &k0, &k1);
-        UncountSink undo{hist, p, {}};
+        UncountSink undo{hist, p, ncells_all, {}};
         (void)walk_chunk(B, P, row, k0, k1, start[g], undo);
         undo.flush();
         StageSink redo{0, B.thread_gbase + thread_of_chunk(P, row, c), cap, stage,
-                       hist, p, {}};
+                       hist, p, ncells_all, {}};
         end[g] = walk_chunk(B, P, row, k0, k1, truth, redo);
         redo.flush();
         count[g] = redo.n;
@@ -543,7 +548,7 @@ __global__ void k_fixup(const Build* __restrict__ builds,
 // into the buckets.
 __global__ void k_scatter(const Build* __restrict__ builds,
                           const int32_t* __restrict__ pair_of_wave,
-                          int64_t nthreads_all,
+                          int64_t nthreads_all, int64_t ncells_all,
                           const int32_t* __restrict__ start,
                           const int32_t* __restrict__ count,
                           const Triple* __restrict__ stage, int64_t cap,
@@ -559,14 +564,14 @@ __global__ void k_scatter(const Build* __restrict__ builds,
     const ChunkRef r = chunk_of_thread(B, t - B.thread_gbase);
     if (!r.valid) return;
     const int64_t g = B.chunk_gbase + r.g;
-    uint32_t* cursor = cursor_all + 4 * B.cell_gbase;
+    uint32_t* cursor = cursor_all + B.cell_gbase;
     const int32_t n = count[g];
     const uint32_t ord = (uint32_t)chunk_off[g];
     if (n > cap) {
         const Pass& P = B.pass[r.p];
         int64_t k0, k1;
         chunk_range(B, P, r.c, &k0, &k1);
-        BucketSink sink{ord, cursor, rec, n_rec, overflow, r.p};
+        BucketSink sink{ord, cursor, rec, n_rec, overflow, r.p, ncells_all};
         (void)walk_chunk(B, P, r.row, k0, k1, start[g], sink);
         atomicAdd(restaged, 1);
         return;
@@ -581,7 +586,7 @@ __global__ void k_scatter(const Build* __restrict__ builds,
         while (q1 < n && q1 - q0 < 16 && stage[stage_index(t, q1, cap)].cin == e0.cin)
             q1++;
         const uint32_t m = (uint32_t)(q1 - q0);
-        const uint32_t pos = atomicAdd(&cursor[4 * (int64_t)e0.cin + r.p], m);
+        const uint32_t pos = atomicAdd(&cursor[r.p * ncells_all + e0.cin], m);
         if ((uint64_t)pos + m <= n_rec) {
             for (int32_t q = q0; q < q1; q++) {
                 const Triple e = q == q0 ? e0 : stage[stage_index(t, q, cap)];
@@ -599,52 +604,102 @@ __global__ void k_scatter(const Build* __restrict__ builds,
 
 // ---- reduce and CSR ------------------------------------------------------
 
-// Coalescing, one wavefront per group of 64 consecutive input cells (whose
-// buckets are one contiguous range).  The range is loaded into LDS with
-// coalesced loads, each lane sorts and reduces its own bucket there
-// (bucket_reduce), and the wave writes the group's unique (out, weight)
-// pairs back, packed from the start of the range, in row order.  A group
-// whose range exceeds the LDS share reduces in global memory instead and is
-// marked so the CSR pass copies it bucket by bucket.
+// Coalescing, one wavefront per group of 64 consecutive input cells.  The
+// group's records sit in four pass ranges (the buckets are pass-major); each
+// lane copies its cell's four pieces into a cell-major LDS layout.  Each
+// lane divides its cell's terms by the cell area, sorts them and reduces
+// equal outputs (bucket_reduce); the wave writes the group's unique
+// (out, weight) pairs packed, in row order, into the cell-major output
+// `red` from the group's first cell offset.  Groups beyond the LDS share go
+// through it in runs of consecutive cells, each cell reduced in place at
+// its own cell offset (the group is then marked unpacked); a single cell
+// beyond the share is gathered and finished in global memory by its lane.
 constexpr int RED_WAVES = 2;     // wavefronts per block
 constexpr int RED_CAP = 1536;    // LDS entries per wavefront (24 KiB)
 
+// pass-range position -> LDS slot, for the cells of one wavefront's group
+struct RedTables {
+    int32_t offp[4][65];  // per pass: cell l's first position in the range
+    int32_t tab[4][64];   // per pass: LDS slot of position q is q + tab[p][l]
+};
+
+// Load the records of cells [c0, c1) (group-local) into LDS slots
+// (tab[p][l] + q - shift): each lane copies its own cell's piece of every
+// pass range (the pieces of neighbouring lanes are neighbouring in memory).
+__device__ inline void load_cells(const KV* __restrict__ rec, const int64_t* ps,
+                                  const RedTables& T, int c0, int c1, int32_t shift,
+                                  uint64_t* K, double* V, int lane)
+{
+    if (lane < c0 || lane >= c1) return;
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        const int32_t q0 = T.offp[p][lane], q1 = T.offp[p][lane + 1];
+        const KV* src = rec + ps[p];
+        const int32_t d = T.tab[p][lane] - shift;
+        for (int32_t q = q0; q < q1; q++) {
+            const KV r = src[q];
+            K[q + d] = r.key;
+            V[q + d] = r.val;
+        }
+    }
+}
+
+__device__ inline void wave_sync_lds()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
-    const int64_t* __restrict__ bucket_off, KV* __restrict__ rec,
-    const double* __restrict__ vol_in, const double* __restrict__ wi,
-    int64_t n_in, int32_t* __restrict__ nnz_row,
+    const int64_t* __restrict__ bucket_off, int64_t n_in,
+    const int64_t* __restrict__ cell_off, const KV* __restrict__ rec,
+    KV* __restrict__ red, const double* __restrict__ vol_in,
+    const double* __restrict__ wi, int32_t* __restrict__ nnz_row,
     int32_t* __restrict__ group_mode, int32_t* __restrict__ long_list,
     int32_t* __restrict__ n_long)
 {
     __shared__ uint64_t sk[RED_WAVES][RED_CAP];
     __shared__ double sv[RED_WAVES][RED_CAP];
+    __shared__ RedTables tabs[RED_WAVES];
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t grp = (int64_t)blockIdx.x * RED_WAVES + wid;
     const int64_t i0 = grp * 64;
     if (i0 >= n_in) return;  // uniform per wavefront; no block barriers below
     const int64_t i1 = i0 + 64 < n_in ? i0 + 64 : n_in;
+    const int nl = (int)(i1 - i0);
     const int64_t i = i0 + lane;
-    const int64_t base = bucket_off[4 * i0];
-    const int64_t total = bucket_off[4 * i1] - base;
+    const int64_t base = cell_off[i0];
+    const int64_t total = cell_off[i1] - base;
+    RedTables& T = tabs[wid];
+    int64_t ps[4];
     int64_t s = 0, n = 0;
     double vol = 1;
     if (i < i1) {
-        s = bucket_off[4 * i] - base;
-        n = bucket_off[4 * i + 4] - bucket_off[4 * i];
+        s = cell_off[i] - base;
+        n = cell_off[i + 1] - cell_off[i];
         vol = vol_in[i];
     }
+    int32_t pre = 0;
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        ps[p] = bucket_off[p * n_in + i0];
+        if (i < i1) {
+            const int64_t b = bucket_off[p * n_in + i];
+            const int32_t o = (int32_t)(b - ps[p]);
+            T.offp[p][lane] = o;
+            T.tab[p][lane] = (int32_t)s + pre - o;
+            pre += (int32_t)(bucket_off[p * n_in + i + 1] - b);
+        }
+        if (lane == 0) T.offp[p][nl] = (int32_t)(bucket_off[p * n_in + i1] - ps[p]);
+    }
+    wave_sync_lds();
+    uint64_t* K = sk[wid];
+    double* V = sv[wid];
     bool lr = false;
     if (total <= RED_CAP) {
-        uint64_t* K = sk[wid];
-        double* V = sv[wid];
-        for (int64_t q = lane; q < total; q += 64) {
-            const KV r = rec[base + q];
-            K[q] = r.key;
-            V[q] = r.val;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        load_cells(rec, ps, T, 0, nl, 0, K, V, lane);
+        wave_sync_lds();
         int64_t m = 0;
         if (i < i1) {
             for (int64_t q = 0; q < n; q++) V[s + q] = weight_of(V[s + q], vol, wi, i);
@@ -652,7 +707,7 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
         }
         if (lr) m = n;  // left sorted, uncompacted, for k_bucket_long
         const bool packed = __ballot(lr) == 0;
-        int64_t dst = s;  // a group with a long run keeps bucket starts
+        int64_t dst = s;  // a group with a long run keeps cell starts
         if (packed) {
             // exclusive prefix of m over the wave: packed destination
             int64_t incl = m;
@@ -667,7 +722,7 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
             KV r;
             r.key = K[s + t];
             r.val = V[s + t];
-            rec[base + dst + t] = r;
+            red[base + dst + t] = r;
         }
         if (i < i1) {
             nnz_row[i] = (int32_t)m;
@@ -675,20 +730,21 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
         }
         if (lane == 0) group_mode[grp] = packed ? 0 : 1;
     } else {
-        // Oversized group: the cells go through the LDS in runs of
-        // consecutive cells that fit it, each cell reduced in place at its
-        // bucket start (the group is not packed).  A single cell larger
-        // than the LDS share is finished in global memory by its lane.
-        uint64_t* K = sk[wid];
-        double* V = sv[wid];
-        int64_t c = i0;
-        while (c < i1) {
-            const int64_t sb = bucket_off[4 * c] - base;  // wave-uniform
-            const bool in = i >= c && i < i1 && s + n - sb <= RED_CAP;
+        int c = 0;  // group-local first cell of the next run
+        while (c < nl) {
+            const int64_t sb = cell_off[i0 + c] - base;  // wave-uniform
+            const bool in = lane >= c && i < i1 && s + n - sb <= RED_CAP;
             const uint64_t mask = __ballot(in);
             if (mask == 0) {
-                if (i == c) {
-                    KV* r = rec + base + s;
+                if (lane == c) {
+                    // gather the cell's four pass pieces, then finish here
+                    KV* r = red + base + s;
+                    int64_t k = 0;
+                    for (int p = 0; p < 4; p++) {
+                        const int64_t b0 = bucket_off[p * n_in + i];
+                        const int64_t b1 = bucket_off[p * n_in + i + 1];
+                        for (int64_t q = b0; q < b1; q++) r[k++] = rec[q];
+                    }
                     for (int64_t q = 0; q < n; q++)
                         r[q].val = weight_of(r[q].val, vol, wi, i);
                     sort_records(r, n);
@@ -697,46 +753,50 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
                 c++;
                 continue;
             }
-            const int64_t d = c + __popcll(mask);  // cells [c, d) fit
-            const int64_t tot = bucket_off[4 * d] - base - sb;
-            for (int64_t q = lane; q < tot; q += 64) {
-                const KV r = rec[base + sb + q];
-                K[q] = r.key;
-                V[q] = r.val;
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const int d = c + __popcll(mask);  // cells [c, d) fit
+            load_cells(rec, ps, T, c, d, (int32_t)sb, K, V, lane);
+            wave_sync_lds();
             if (in) {
                 uint64_t* Kc = K + (s - sb);
                 double* Vc = V + (s - sb);
                 for (int64_t q = 0; q < n; q++) Vc[q] = weight_of(Vc[q], vol, wi, i);
-                bool lr = false;
-                int64_t m = bucket_reduce(Kc, Vc, n, &lr);
-                if (lr) m = n;  // left sorted, uncompacted, for k_bucket_long
+                bool lrc = false;
+                int64_t m = bucket_reduce(Kc, Vc, n, &lrc);
+                if (lrc) m = n;  // left sorted, uncompacted, for k_bucket_long
                 for (int64_t t = 0; t < m; t++) {
                     KV r;
                     r.key = Kc[t];
                     r.val = Vc[t];
-                    rec[base + s + t] = r;
+                    red[base + s + t] = r;
                 }
                 nnz_row[i] = (int32_t)m;
-                if (lr) long_list[atomicAdd(n_long, 1)] = (int32_t)i;
+                if (lrc) long_list[atomicAdd(n_long, 1)] = (int32_t)i;
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            wave_sync_lds();
             c = d;
         }
         if (lane == 0) group_mode[grp] = 1;
     }
 }
 
-// Buckets with a run of more than 129 terms (see bucket_reduce): left
-// sorted at their bucket start (their group is never packed) and finished
-// here.  Grid-stride over the normally empty list; fixes nnz_row.
-__global__ void k_bucket_long(const int64_t* __restrict__ bucket_off,
-                              KV* __restrict__ rec,
+// Records of one input cell (all passes): the per-cell counts that give
+// the cell-major output offsets.
+__global__ void k_cell_counts(const int64_t* __restrict__ bucket_off, int64_t n_in,
+                              int32_t* __restrict__ cnt)
+{
+    const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (c >= n_in) return;
+    int64_t t = 0;
+#pragma unroll
+    for (int p = 0; p < 4; p++) t += bucket_off[p * n_in + c + 1] - bucket_off[p * n_in + c];
+    cnt[c] = (int32_t)t;
+}
+
+// Cells with a run of more than 129 terms (see bucket_reduce): left sorted
+// at their cell offset (their group is never packed) and finished here.
+// Grid-stride over the normally empty list; fixes nnz_row.
+__global__ void k_bucket_long(const int64_t* __restrict__ cell_off,
+                              KV* __restrict__ red,
                               const int32_t* __restrict__ long_list,
                               const int32_t* __restrict__ n_long,
                               int32_t* __restrict__ nnz_row)
@@ -745,9 +805,8 @@ __global__ void k_bucket_long(const int64_t* __restrict__ bucket_off,
     for (int32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nl;
          q += gridDim.x * blockDim.x) {
         const int64_t i = long_list[q];
-        const int64_t start = bucket_off[4 * i];
-        nnz_row[i] = (int32_t)bucket_reduce_long(rec + start,
-                                                 bucket_off[4 * i + 4] - start);
+        const int64_t start = cell_off[i];
+        nnz_row[i] = (int32_t)bucket_reduce_long(red + start, cell_off[i + 1] - start);
     }
 }
 
@@ -759,7 +818,7 @@ __global__ void k_bucket_long(const int64_t* __restrict__ bucket_off,
 __global__ void k_csr_groups(const PairCells* __restrict__ pcs,
                              const int32_t* __restrict__ pair_of_group,
                              int64_t n_groups,
-                             const int64_t* __restrict__ bucket_off,
+                             const int64_t* __restrict__ cell_off,
                              const int64_t* __restrict__ row_off,
                              const int32_t* __restrict__ group_mode,
                              const KV* __restrict__ rec,
@@ -784,7 +843,7 @@ __global__ void k_csr_groups(const PairCells* __restrict__ pcs,
     int32_t* hist = hist_out + P.out_gbase;
     const int64_t d0 = row_off[i0];
     if (group_mode[grp] == 0) {
-        const int64_t b0 = bucket_off[4 * i0], cnt = row_off[i1] - d0;
+        const int64_t b0 = cell_off[i0], cnt = row_off[i1] - d0;
         for (int64_t q = lane; q < cnt; q += 64) {
             const KV r = rec[b0 + q];
             const int32_t o = (int32_t)r.key;
@@ -793,7 +852,7 @@ __global__ void k_csr_groups(const PairCells* __restrict__ pcs,
             atomicAdd(&hist[o], 1);
         }
     } else if (i < i1) {
-        const int64_t sb = bucket_off[4 * i], d = row_off[i], m = row_off[i + 1] - d;
+        const int64_t sb = cell_off[i], d = row_off[i], m = row_off[i + 1] - d;
         for (int64_t q = 0; q < m; q++) {
             const KV r = rec[sb + q];
             const int32_t o = (int32_t)r.key;
@@ -1269,14 +1328,14 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
     DevBuf<Triple> stage;
     RG_TRY(stage.alloc((size_t)thr_tot * cap, s));
     k_walk_count<<<grid_for(thr_tot, block), block, 0, s>>>(
-        d_builds, pair_of_wave.p, thr_tot, start.p, end.p, count.p, hist.p,
+        d_builds, pair_of_wave.p, thr_tot, cell_tot, start.p, end.p, count.p, hist.p,
         stage.p, cap);
     RG_HIP(hipGetLastError());
     tm.mark();  // 3
 
     // ---- fix-up
     k_fixup<<<grid_for(row_tot, 64), 64, 0, s>>>(
-        d_builds, pair_of_row.p, row_tot, start.p, end.p, count.p, hist.p,
+        d_builds, pair_of_row.p, row_tot, cell_tot, start.p, end.p, count.p, hist.p,
         counters.p + 1, counters.p + 2, stage.p, cap);
     RG_HIP(hipGetLastError());
     DevBuf<int64_t> chunk_off, bucket_off;
@@ -1311,7 +1370,7 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
                                                               4 * cell_tot);
     tm.mark();  // 5
     k_scatter<<<grid_for(thr_tot, block), block, 0, s>>>(
-        d_builds, pair_of_wave.p, thr_tot, start.p, count.p, stage.p, cap,
+        d_builds, pair_of_wave.p, thr_tot, cell_tot, start.p, count.p, stage.p, cap,
         chunk_off.p, cursor, rec.p, (uint32_t)T, counters.p + 2, counters.p + 4);
     RG_HIP(hipGetLastError());
     tm.mark();  // 6
@@ -1322,18 +1381,26 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
 
     // ---- reduce -> CSR by input
     DevBuf<int32_t> nnz_row, long_list, group_mode;
-    DevBuf<int64_t> row_off;
+    DevBuf<int64_t> row_off, cell_off;
+    DevBuf<KV> red;
     const int64_t n_groups = cell_tot / 64;
     RG_TRY(nnz_row.alloc(cell_tot, s));
     RG_TRY(long_list.alloc(cell_tot, s));
     RG_TRY(group_mode.alloc(n_groups, s));
     RG_TRY(row_off.alloc(cell_tot + 1, s));
+    RG_TRY(cell_off.alloc(cell_tot + 1, s));
+    RG_TRY(red.alloc(T, s));
+    // cell-major offsets of the reduction's output (nnz_row as scratch)
+    k_cell_counts<<<grid_for(cell_tot, 256), 256, 0, s>>>(bucket_off.p, cell_tot,
+                                                          nnz_row.p);
+    RG_HIP(hipGetLastError());
+    RG_TRY(scan_i32_to_i64(nnz_row.p, cell_off.p, cell_tot, s));
     tm.mark();  // 7
     k_reduce_groups<<<(unsigned)((n_groups + RED_WAVES - 1) / RED_WAVES),
-                      RED_WAVES * 64, 0, s>>>(bucket_off.p, rec.p, vol.p, wi_all.p,
-                                              cell_tot, nnz_row.p, group_mode.p,
-                                              long_list.p, counters.p + 3);
-    k_bucket_long<<<4, 64, 0, s>>>(bucket_off.p, rec.p, long_list.p,
+                      RED_WAVES * 64, 0, s>>>(bucket_off.p, cell_tot, cell_off.p, rec.p,
+                                              red.p, vol.p, wi_all.p, nnz_row.p,
+                                              group_mode.p, long_list.p, counters.p + 3);
+    k_bucket_long<<<4, 64, 0, s>>>(cell_off.p, red.p, long_list.p,
                                    counters.p + 3, nnz_row.p);
     RG_HIP(hipGetLastError());
     tm.mark();  // 8
@@ -1370,9 +1437,10 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
     RG_TRY(hist_out.alloc(out_tot, s));
     RG_TRY(off_out.alloc(out_tot + 1, s));
     RG_TRY(fill_zero(hist_out.p, out_tot, s));
+    rec.reset();
     k_csr_groups<<<grid_for(n_groups * 64, 256), 256, 0, s>>>(
-        d_pcs, pair_of_ing.p, n_groups, bucket_off.p, row_off.p, group_mode.p,
-        rec.p, rp_in.p, c_in.p, v_in.p, hist_out.p);
+        d_pcs, pair_of_ing.p, n_groups, cell_off.p, row_off.p, group_mode.p,
+        red.p, rp_in.p, c_in.p, v_in.p, hist_out.p);
     RG_HIP(hipGetLastError());
     RG_TRY(scan_i32_to_i64(hist_out.p, off_out.p, out_tot, s));
     RG_TRY(fill_zero(hist_out.p, out_tot, s));  // cursor
