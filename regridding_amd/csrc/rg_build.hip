@@ -613,9 +613,9 @@ __global__ void k_scatter(const Build* __restrict__ builds,
 // `red` from the group's first cell offset.  Groups beyond the LDS share go
 // through it in runs of consecutive cells, each cell reduced in place at
 // its own cell offset (the group is then marked unpacked); a single cell
-// beyond the share is gathered and finished in global memory by its lane.
+// beyond the share is left to k_bucket_long.
 constexpr int RED_WAVES = 2;     // wavefronts per block
-constexpr int RED_CAP = 1536;    // LDS entries per wavefront (24 KiB)
+constexpr int RED_CAP = 1024;    // LDS entries per wavefront (16 KiB)
 
 // pass-range position -> LDS slot, for the cells of one wavefront's group
 struct RedTables {
@@ -736,20 +736,9 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
             const bool in = lane >= c && i < i1 && s + n - sb <= RED_CAP;
             const uint64_t mask = __ballot(in);
             if (mask == 0) {
-                if (lane == c) {
-                    // gather the cell's four pass pieces, then finish here
-                    KV* r = red + base + s;
-                    int64_t k = 0;
-                    for (int p = 0; p < 4; p++) {
-                        const int64_t b0 = bucket_off[p * n_in + i];
-                        const int64_t b1 = bucket_off[p * n_in + i + 1];
-                        for (int64_t q = b0; q < b1; q++) r[k++] = rec[q];
-                    }
-                    for (int64_t q = 0; q < n; q++)
-                        r[q].val = weight_of(r[q].val, vol, wi, i);
-                    sort_records(r, n);
-                    nnz_row[i] = (int32_t)bucket_reduce_long(r, n);
-                }
+                // a cell beyond the LDS share: gathered and finished by
+                // k_bucket_long (flagged in the long list)
+                if (lane == c) long_list[atomicAdd(n_long, 1)] = (int32_t)i | INT32_MIN;
                 c++;
                 continue;
             }
@@ -792,11 +781,16 @@ __global__ void k_cell_counts(const int64_t* __restrict__ bucket_off, int64_t n_
     cnt[c] = (int32_t)t;
 }
 
-// Cells with a run of more than 129 terms (see bucket_reduce): left sorted
-// at their cell offset (their group is never packed) and finished here.
-// Grid-stride over the normally empty list; fixes nnz_row.
-__global__ void k_bucket_long(const int64_t* __restrict__ cell_off,
-                              KV* __restrict__ red,
+// Cells with a run of more than 129 terms (see bucket_reduce), left sorted
+// at their cell offset by k_reduce_groups, and cells beyond its LDS share
+// (flagged with the sign bit: gathered from their four pass pieces, divided
+// and sorted here first), finished in global memory.  Grid-stride over the
+// normally empty list; fixes nnz_row.
+__global__ void k_bucket_long(const int64_t* __restrict__ bucket_off, int64_t n_in,
+                              const int64_t* __restrict__ cell_off,
+                              const KV* __restrict__ rec, KV* __restrict__ red,
+                              const double* __restrict__ vol_in,
+                              const double* __restrict__ wi,
                               const int32_t* __restrict__ long_list,
                               const int32_t* __restrict__ n_long,
                               int32_t* __restrict__ nnz_row)
@@ -804,9 +798,22 @@ __global__ void k_bucket_long(const int64_t* __restrict__ cell_off,
     const int32_t nl = *n_long;
     for (int32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nl;
          q += gridDim.x * blockDim.x) {
-        const int64_t i = long_list[q];
-        const int64_t start = cell_off[i];
-        nnz_row[i] = (int32_t)bucket_reduce_long(red + start, cell_off[i + 1] - start);
+        const int32_t e = long_list[q];
+        const int64_t i = e & INT32_MAX;
+        const int64_t start = cell_off[i], n = cell_off[i + 1] - start;
+        KV* r = red + start;
+        if (e < 0) {
+            int64_t k = 0;
+            for (int p = 0; p < 4; p++) {
+                const int64_t b0 = bucket_off[p * n_in + i];
+                const int64_t b1 = bucket_off[p * n_in + i + 1];
+                for (int64_t t = b0; t < b1; t++) r[k++] = rec[t];
+            }
+            const double vol = vol_in[i];
+            for (int64_t t = 0; t < n; t++) r[t].val = weight_of(r[t].val, vol, wi, i);
+            sort_records(r, n);
+        }
+        nnz_row[i] = (int32_t)bucket_reduce_long(r, n);
     }
 }
 
@@ -897,8 +904,8 @@ __global__ void k_transpose_scatter(const PairCells* __restrict__ pcs,
 // per lane in LDS, stored back coalesced.  Oversized groups sort in global
 // memory.  Also writes each pair's int32 row pointers (pair k's n_out + 1
 // pointers start at rowptr_all[out_gbase + k]).
-constexpr int SORT_WAVES = 4;
-constexpr int SORT_CAP = 1536;  // entries per wavefront (18 KiB)
+constexpr int SORT_WAVES = 2;
+constexpr int SORT_CAP = 1024;  // entries per wavefront (12 KiB)
 
 __global__ __launch_bounds__(SORT_WAVES * 64) void k_rows_sort(
     const PairCells* __restrict__ pcs, const int32_t* __restrict__ pair_of_group,
@@ -1400,8 +1407,9 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
                       RED_WAVES * 64, 0, s>>>(bucket_off.p, cell_tot, cell_off.p, rec.p,
                                               red.p, vol.p, wi_all.p, nnz_row.p,
                                               group_mode.p, long_list.p, counters.p + 3);
-    k_bucket_long<<<4, 64, 0, s>>>(cell_off.p, red.p, long_list.p,
-                                   counters.p + 3, nnz_row.p);
+    k_bucket_long<<<4, 64, 0, s>>>(bucket_off.p, cell_tot, cell_off.p, rec.p, red.p,
+                                   vol.p, wi_all.p, long_list.p, counters.p + 3,
+                                   nnz_row.p);
     RG_HIP(hipGetLastError());
     tm.mark();  // 8
     RG_TRY(scan_i32_to_i64(nnz_row.p, row_off.p, cell_tot, s));
