@@ -397,6 +397,7 @@ def bench_batched(args) -> dict:
     ms_batched = timed(batched, 3)
     hs = batched(timings=True)
     st = hs[0].stats()
+    log(f"batched build: {st['n_long_cells']} input cells reduced in global memory")
     triples = sum(h.stats()["n_triples"] for h in hs)
     nnz = sum(h.nnz for h in hs)
     for h in hs:

@@ -40,7 +40,8 @@ typedef struct rg_weights* rg_weights_t;
 /* Tuning of the build.  Zero fields select the defaults. */
 typedef struct rg_tuning {
     int32_t edges_per_lane; /* sweep edges walked by one lane (default 16) */
-    int32_t block;          /* threads per block of the walk kernels (128) */
+    int32_t block;          /* threads per block of the walk kernels (128; a
+                               multiple of 64, at most 512) */
     int32_t flags;          /* RG_TUNE_TIMINGS: record per-phase times */
     int32_t stage_per_edge; /* triples staged per sweep edge by the counting
                                walk (default 8); chunks that emit more are
@@ -58,6 +59,9 @@ typedef struct rg_build_stats {
     int64_t n_search_anomalies; /* row starts inside the boundary polygon
                                    that the cell search did not place */
     int64_t n_restaged;     /* chunks whose triples overflowed the staging */
+    int64_t n_long_cells;   /* input cells reduced in global memory (more
+                               records than the LDS share, or a run of more
+                               than 129 terms) */
     /* with RG_TUNE_TIMINGS, stream times (HIP events) of: prepare (areas,
      * static-grid tables), start (row and chunk start states), count (the
      * counting walk kernel), fixup (fix-up walk, scans and the host wait

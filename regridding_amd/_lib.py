@@ -118,6 +118,7 @@ class rg_build_stats(ctypes.Structure):
         ("n_subsegments", ctypes.c_int64),
         ("n_search_anomalies", ctypes.c_int64),
         ("n_restaged", ctypes.c_int64),
+        ("n_long_cells", ctypes.c_int64),
         ("ms_prepare", ctypes.c_double),
         ("ms_start", ctypes.c_double),
         ("ms_count", ctypes.c_double),

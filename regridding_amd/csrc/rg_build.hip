@@ -440,6 +440,11 @@ __global__ void k_row_start(const Build* __restrict__ builds,
     if (lane == 0) start[g] = state;
 }
 
+// The walk kernels take blocks of up to 512 threads; the bound (rather than
+// the default 1024) lets the walk keep its state in registers without
+// spilling.
+constexpr int WALK_MAX_BLOCK = 512;
+
 // The pair of walk thread t of a batched build (wavefront-uniform: every
 // pair's walk threads are whole wavefronts).
 __device__ inline const Build& build_of_thread(const Build* __restrict__ builds,
@@ -449,7 +454,7 @@ __device__ inline const Build& build_of_thread(const Build* __restrict__ builds,
     return builds[__builtin_amdgcn_readfirstlane(pair_of_wave[t >> 6])];
 }
 
-__global__ void k_chunk_start(const Build* __restrict__ builds,
+__global__ __launch_bounds__(WALK_MAX_BLOCK) void k_chunk_start(const Build* __restrict__ builds,
                               const int32_t* __restrict__ pair_of_wave,
                               int64_t nthreads_all, int32_t* __restrict__ start)
 {
@@ -472,7 +477,7 @@ __device__ inline void chunk_range(const Build& B, const Pass& P, int64_t c,
     if (*k1 < *k0) *k1 = *k0;
 }
 
-__global__ void k_walk_count(const Build* __restrict__ builds,
+__global__ __launch_bounds__(WALK_MAX_BLOCK) void k_walk_count(const Build* __restrict__ builds,
                              const int32_t* __restrict__ pair_of_wave,
                              int64_t nthreads_all, int64_t ncells_all,
                              const int32_t* __restrict__ start,
@@ -497,7 +502,7 @@ __global__ void k_walk_count(const Build* __restrict__ builds,
 }
 
 // One thread per row: make every chunk start where its predecessor ended.
-__global__ void k_fixup(const Build* __restrict__ builds,
+__global__ __launch_bounds__(WALK_MAX_BLOCK) void k_fixup(const Build* __restrict__ builds,
                         const int32_t* __restrict__ pair_of_row,
                         int64_t nrows_all, int64_t ncells_all,
                         int32_t* __restrict__ start,
@@ -546,7 +551,7 @@ __global__ void k_fixup(const Build* __restrict__ builds,
 // ordinal (the chunk's first ordinal plus its position in the chunk).
 // Chunks whose triples overflowed the staging walk again, emitting straight
 // into the buckets.
-__global__ void k_scatter(const Build* __restrict__ builds,
+__global__ __launch_bounds__(WALK_MAX_BLOCK) void k_scatter(const Build* __restrict__ builds,
                           const int32_t* __restrict__ pair_of_wave,
                           int64_t nthreads_all, int64_t ncells_all,
                           const int32_t* __restrict__ start,
@@ -1102,8 +1107,8 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
         if (tuning->stage_per_edge > 0) stage_per_edge = tuning->stage_per_edge;
         timings = (tuning->flags & RG_TUNE_TIMINGS) != 0;
     }
-    if (block % 64 != 0 || block > 1024)
-        return fail(RG_ERR_ARGUMENT, "block must be a multiple of 64, <= 1024");
+    if (block % 64 != 0 || block > WALK_MAX_BLOCK)
+        return fail(RG_ERR_ARGUMENT, "block must be a multiple of 64, <= 512");
 
     PhaseTimer tm(timings, s);
     tm.mark();  // 0
@@ -1407,7 +1412,7 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
                       RED_WAVES * 64, 0, s>>>(bucket_off.p, cell_tot, cell_off.p, rec.p,
                                               red.p, vol.p, wi_all.p, nnz_row.p,
                                               group_mode.p, long_list.p, counters.p + 3);
-    k_bucket_long<<<4, 64, 0, s>>>(bucket_off.p, cell_tot, cell_off.p, rec.p, red.p,
+    k_bucket_long<<<256, 64, 0, s>>>(bucket_off.p, cell_tot, cell_off.p, rec.p, red.p,
                                    vol.p, wi_all.p, long_list.p, counters.p + 3,
                                    nnz_row.p);
     RG_HIP(hipGetLastError());
@@ -1478,6 +1483,7 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
     batch.n_fixups = cnt[1];
     batch.n_search_anomalies = cnt[0];
     batch.n_restaged = cnt2[4];
+    batch.n_long_cells = cnt2[3];
     if (timings) {
         batch.ms_prepare = tm.ms(0, 1);
         batch.ms_start = tm.ms(1, 2);
