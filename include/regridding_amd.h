@@ -297,6 +297,22 @@ int rg_multilinear_1d(const double* x_input, int64_t n_input, const double* x_ou
  * clears pending injections. */
 int rg_inject_failure(const char* site);
 
+/* Replaces _fill/_gauss_seidel.py:83-139 (_fill_gauss_seidel_2d): num_iterations
+ * red-black Gauss-Seidel sweeps, in place, over T elements of (ny, nx) values
+ * with periodic neighbours, updating the cells where where != 0; the same
+ * update order as the reference's sequential sweep. */
+int rg_fill_gauss_seidel_2d(double* a, const uint8_t* where, int64_t T, int64_t ny,
+                            int64_t nx, int64_t num_iterations, void* stream);
+
+/* Replace _interp_ndarray.py:172-187 and :190-205
+ * (_ndarray_linear_interpolation_1d / _2d): D elements of a (n) or (nx, ny)
+ * evaluated at m fractional indices each (x, or x and y), clamped to the
+ * last cell; out is (D, m). */
+int rg_interp_linear_1d(const double* a, int64_t n, const double* x, int64_t m,
+                        int64_t D, double* out, void* stream);
+int rg_interp_linear_2d(const double* a, int64_t nx, int64_t ny, const double* x,
+                        const double* y, int64_t m, int64_t D, double* out, void* stream);
+
 /* ---- serving loop (no reference counterpart) ------------------------------
  *
  * A request is one regrid job: a grid pair (or the cache key of weights

@@ -70,6 +70,12 @@ def lib() -> ctypes.CDLL:
             f = getattr(L, name)
             f.restype = None
             f.argtypes = [_pd, _i64, _pd, _i64, _i64, _i64, _pi]
+        L.rgo_fill_gauss_seidel_2d.restype = None
+        L.rgo_fill_gauss_seidel_2d.argtypes = [_pd, ctypes.c_void_p, _i64, _i64, _i64, _i64]
+        L.rgo_interp_linear_1d.restype = None
+        L.rgo_interp_linear_1d.argtypes = [_pd, _i64, _pd, _i64, _i64, _pd]
+        L.rgo_interp_linear_2d.restype = None
+        L.rgo_interp_linear_2d.argtypes = [_pd, _i64, _i64, _pd, _pd, _i64, _i64, _pd]
         L.rgo_set_threads.restype = None
         L.rgo_set_threads.argtypes = [ctypes.c_int]
         _lib = L
@@ -350,3 +356,129 @@ def multilinear_1d(indices, x_input, x_output, weights_input=None, bounds_nan=Fa
         ii[2 * i], io[2 * i], w[2 * i] = a, i, w0
         ii[2 * i + 1], io[2 * i + 1], w[2 * i + 1] = a + 1, i, w1
     return ii, io, w, int(outside.sum())
+
+
+def fill_gauss_seidel_2d(a, where, num_iterations: int) -> np.ndarray:
+    """``_fill_gauss_seidel_2d`` (``_fill/_gauss_seidel.py:83-139``) on
+    (T, ny, nx) arrays: the sequential red-black sweep, in place on a copy."""
+    a = np.array(a, dtype=np.float64, order="C", copy=True)
+    w = np.ascontiguousarray(where, dtype=np.uint8)
+    T, ny, nx = a.shape
+    lib().rgo_fill_gauss_seidel_2d(_d(a), w.ctypes.data, T, ny, nx, int(num_iterations))
+    return a
+
+
+def fill_gauss_seidel_2d_py(a, where, num_iterations: int) -> np.ndarray:
+    """Pure-Python loops of the same sweep, for tiny inputs."""
+    a = np.array(a, dtype=np.float64, copy=True)
+    T, ny, nx = a.shape
+    dx, dy = 2 / (nx - 1), 2 / (ny - 1)
+    dxxinv, dyyinv = 1 / (dx * dx), 1 / (dy * dy)
+    dcent = 1 / (2 * (dxxinv + dyyinv))
+    for t in range(T):
+        for _ in range(num_iterations):
+            for is_odd in (0, 1):
+                for j in range(ny):
+                    for i in range(nx):
+                        if (i + j) & 1 == is_odd and where[t, j, i]:
+                            xterm = dxxinv * (a[t, j, (i - 1) % nx] + a[t, j, (i + 1) % nx])
+                            yterm = dyyinv * (a[t, (j - 1) % ny, i] + a[t, (j + 1) % ny, i])
+                            a[t, j, i] = (xterm + yterm) * dcent
+    return a
+
+
+def fill(a, where=None, axis=None, guess=None, num_iterations: int = 100) -> np.ndarray:
+    """``fill_gauss_seidel`` (``_fill/_gauss_seidel.py:12-80``) with the
+    sweep above."""
+    if where is None:
+        where = np.isnan(a)
+    a = np.array(a, copy=True)
+    a, where = np.broadcast_arrays(a, where, subok=True)
+    a = np.array(a)
+    axis = tuple(range(a.ndim)) if axis is None else np.atleast_1d(axis).tolist()
+    axis = tuple(ax % a.ndim for ax in axis)
+    if guess is None:
+        b = np.where(where, np.nan, a)
+        import warnings
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore", category=RuntimeWarning)
+            med = np.nanmedian(b, axis=axis, keepdims=True)
+        guess = np.where(np.isnan(med), 0, med)
+    a[where] = np.broadcast_to(guess, a.shape)[where]
+    last = (-2, -1)
+    am = np.moveaxis(a, axis, last)
+    wm = np.moveaxis(where, axis, last)
+    shp = am.shape
+    r = fill_gauss_seidel_2d(am.reshape(-1, *shp[-2:]), wm.reshape(-1, *shp[-2:]),
+                             num_iterations)
+    return np.moveaxis(r.reshape(shp), last, axis)
+
+
+def interp_linear_1d(a, x) -> np.ndarray:
+    """``_linear_interpolation`` (``_interp_ndarray.py:208-224``) for every
+    row of ``a`` (D, n) at the points ``x`` (D, m)."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    D, n = a.shape
+    m = x.shape[1]
+    out = np.empty((D, m))
+    lib().rgo_interp_linear_1d(_d(a), n, _d(x), m, D, _d(out))
+    return out
+
+
+def interp_linear_2d(a, x, y) -> np.ndarray:
+    """``_bilinear_interpolation`` (``_interp_ndarray.py:227-269``) for every
+    array of ``a`` (D, nx, ny) at the points ``x``, ``y`` (D, m)."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    D, nx, ny = a.shape
+    m = x.shape[1]
+    out = np.empty((D, m))
+    lib().rgo_interp_linear_2d(_d(a), nx, ny, _d(x), _d(y), m, D, _d(out))
+    return out
+
+
+def ndarray_linear_interpolation(a, indices, axis=None, axis_indices=None) -> np.ndarray:
+    """``ndarray_linear_interpolation`` (``_interp_ndarray.py:11-165``): the
+    reference's per-orthogonal-index loop around the two evaluators."""
+    a = np.asarray(a)
+    shape_a, ndim_a = a.shape, a.ndim
+    bi = np.broadcast(*indices)
+    shape_indices, ndim_indices = bi.shape, bi.ndim
+    if axis is None:
+        axis = tuple(range(ndim_a))
+    axis = np.lib.array_utils.normalize_axis_tuple(axis, ndim=ndim_a)
+    axis = tuple(~(~np.array(axis) % ndim_a))
+    if axis_indices is None:
+        axis_indices = tuple(range(ndim_indices))
+    axis_indices = np.lib.array_utils.normalize_axis_tuple(axis_indices, ndim=ndim_indices)
+    axis_indices = tuple(~(~np.array(axis_indices) % ndim_indices))
+    if len(indices) != len(axis):
+        raise ValueError("The number of indices must match the number of elements in axis")
+    oa = tuple(ax for ax in range(-ndim_a, 0) if ax not in axis)
+    oi = tuple(ax for ax in range(-ndim_indices, 0) if ax not in axis_indices)
+    so = np.broadcast_shapes(tuple(shape_a[ax] for ax in oa),
+                             tuple(shape_indices[ax] for ax in oi))
+    nba, nbi = len(so) + len(axis), len(so) + len(axis_indices)
+    sba = tuple(shape_a[ax] if ax in axis else so[oa.index(ax)] for ax in range(-nba, 0))
+    sbi = tuple(shape_indices[ax] if ax in axis_indices else so[oi.index(ax)]
+                for ax in range(-nbi, 0))
+    a = np.broadcast_to(a, sba)
+    indices = tuple(np.broadcast_to(ind, sbi) for ind in indices)
+    result = np.empty(sbi)
+    for index in np.ndindex(*so):
+        ia = tuple(slice(None) if ax in axis else index[oa.index(ax)] for ax in range(-nba, 0))
+        ii = tuple(slice(None) if ax in axis_indices else index[oi.index(ax)]
+                   for ax in range(-nbi, 0))
+        sub = a[ia]
+        shp = indices[0][ii].shape
+        if len(axis) == 1:
+            r = interp_linear_1d(sub[None], indices[0][ii].reshape(1, -1))
+        elif len(axis) == 2:
+            r = interp_linear_2d(sub[None], indices[0][ii].reshape(1, -1),
+                                 indices[1][ii].reshape(1, -1))
+        else:
+            raise NotImplementedError
+        result[ii] = r.reshape(shp)
+    return result

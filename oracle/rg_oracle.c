@@ -871,3 +871,75 @@ void rgo_find_indices_searchsorted_1d(const double *xin, int64_t m,
         }
     }
 }
+
+/* _fill/_gauss_seidel.py:83-139: red-black Gauss-Seidel, each colour swept
+ * row-major with periodic neighbours, T independent (ny, nx) slices */
+void rgo_fill_gauss_seidel_2d(double *a, const uint8_t *where, int64_t T,
+                              int64_t ny, int64_t nx, int64_t num_iterations)
+{
+    const double xmin = -1, xmax = 1, ymin = -1, ymax = 1;
+    const double dx = (xmax - xmin) / (double)(nx - 1);
+    const double dy = (ymax - ymin) / (double)(ny - 1);
+    const double dxxinv = 1 / (dx * dx), dyyinv = 1 / (dy * dy);
+    const double dcent = 1 / (2 * (dxxinv + dyyinv));
+    for (int64_t t = 0; t < T; t++) {
+        double *at = a + t * ny * nx;
+        const uint8_t *wt = where + t * ny * nx;
+        for (int64_t k = 0; k < num_iterations; k++) {
+            for (int is_odd = 0; is_odd < 2; is_odd++) {
+                for (int64_t j = 0; j < ny; j++) {
+                    for (int64_t i = 0; i < nx; i++) {
+                        if (((i + j) & 1) != is_odd || !wt[j * nx + i]) continue;
+                        int64_t i9 = ((i - 1) % nx + nx) % nx, i1 = (i + 1) % nx;
+                        int64_t j9 = ((j - 1) % ny + ny) % ny, j1 = (j + 1) % ny;
+                        double xterm = dxxinv * (at[j * nx + i9] + at[j * nx + i1]);
+                        double yterm = dyyinv * (at[j9 * nx + i] + at[j1 * nx + i]);
+                        at[j * nx + i] = (xterm + yterm) * dcent;
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* _interp_ndarray.py:208-224 (_linear_interpolation), D rows of n samples,
+ * m points per row */
+void rgo_interp_linear_1d(const double *a, int64_t n, const double *x, int64_t m,
+                          int64_t D, double *out)
+{
+    for (int64_t d = 0; d < D; d++) {
+        for (int64_t q = 0; q < m; q++) {
+            double xv = x[d * m + q];
+            int64_t x0 = (int64_t)floor(xv);
+            if (x0 < 0) x0 = 0;
+            else if (x0 > n - 2) x0 = n - 2;
+            double a0 = a[d * n + x0], a1 = a[d * n + x0 + 1];
+            double dxv = xv - (double)x0;
+            out[d * m + q] = a0 * (1 - dxv) + a1 * dxv;
+        }
+    }
+}
+
+/* _interp_ndarray.py:227-269 (_bilinear_interpolation) in the written order
+ * of its final sum, D arrays of (nx, ny), m points per array */
+void rgo_interp_linear_2d(const double *a, int64_t nx, int64_t ny, const double *x,
+                          const double *y, int64_t m, int64_t D, double *out)
+{
+    for (int64_t d = 0; d < D; d++) {
+        const double *ad = a + d * nx * ny;
+        for (int64_t q = 0; q < m; q++) {
+            double xv = x[d * m + q], yv = y[d * m + q];
+            int64_t x00 = (int64_t)floor(xv), y00 = (int64_t)floor(yv);
+            if (x00 < 0) x00 = 0;
+            else if (x00 > nx - 2) x00 = nx - 2;
+            if (y00 < 0) y00 = 0;
+            else if (y00 > ny - 2) y00 = ny - 2;
+            double a00 = ad[x00 * ny + y00], a01 = ad[x00 * ny + y00 + 1];
+            double a10 = ad[(x00 + 1) * ny + y00], a11 = ad[(x00 + 1) * ny + y00 + 1];
+            double dxv = xv - (double)x00, dyv = yv - (double)y00;
+            double w00 = (1 - dxv) * (1 - dyv), w01 = (1 - dxv) * dyv;
+            double w10 = dxv * (1 - dyv), w11 = dxv * dyv;
+            out[d * m + q] = (a00 * w00) + (a01 * w01) + (a10 * w10) + (a11 * w11);
+        }
+    }
+}

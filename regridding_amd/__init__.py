@@ -14,6 +14,8 @@ from ._weights import weights
 from ._regrid import regrid_from_weights, regrid
 from ._transpose import transpose_weights, transpose_weights_conservative
 from ._find_indices import find_indices
+from ._fill import fill
+from ._interp import ndarray_linear_interpolation
 
 __all__ = [
     "SEED_DEFAULT",
@@ -23,4 +25,6 @@ __all__ = [
     "transpose_weights",
     "transpose_weights_conservative",
     "find_indices",
+    "fill",
+    "ndarray_linear_interpolation",
 ]

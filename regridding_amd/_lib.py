@@ -186,6 +186,9 @@ SIGNATURES = {
     "rg_conservative_1d_emit": (ctypes.c_int, [_vp, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp,
                                                _vp, _vp, _vp]),
     "rg_cell_length_1d": (ctypes.c_int, [_vp, _i64, _i64, _vp, _vp]),
+    "rg_fill_gauss_seidel_2d": (ctypes.c_int, [_vp, _vp, _i64, _i64, _i64, _i64, _vp]),
+    "rg_interp_linear_1d": (ctypes.c_int, [_vp, _i64, _vp, _i64, _i64, _vp, _vp]),
+    "rg_interp_linear_2d": (ctypes.c_int, [_vp, _i64, _i64, _vp, _vp, _i64, _i64, _vp, _vp]),
     "rg_multilinear_1d": (ctypes.c_int, [_vp, _i64, _vp, _i64, _i64, _vp, _vp, ctypes.c_int32,
                                          _vp, _vp, _vp, _pi64, _vp]),
     "rg_weights_info": (ctypes.c_int, [_vp, _pi64, _pi64, _pi64]),
