@@ -16,6 +16,7 @@ from ._transpose import transpose_weights, transpose_weights_conservative
 from ._find_indices import find_indices
 from ._fill import fill
 from ._interp import ndarray_linear_interpolation
+from . import weights_io
 
 __all__ = [
     "SEED_DEFAULT",
