@@ -514,6 +514,7 @@ def bench_serve(args) -> dict:
                 if start is not None:
                     live.append((start, n))
                     return start
+                check_failed()
                 lock.wait()
 
     def ring_give():
@@ -522,16 +523,25 @@ def bench_serve(args) -> dict:
             lock.notify_all()
 
     def completer():
-        while True:
+        try:
+            while True:
+                with lock:
+                    while not inflight and not state["done"]:
+                        lock.wait()
+                    if not inflight and state["done"]:
+                        return
+                    tk = inflight.popleft()
+                _, done_ns = srv.wait(tk)
+                lat.append((done_ns - tk.arrival_ns) * 1e-6)
+                ring_give()
+        except BaseException as e:  # hand the failure to the submitting loop
             with lock:
-                while not inflight and not state["done"]:
-                    lock.wait()
-                if not inflight and state["done"]:
-                    return
-                tk = inflight.popleft()
-            _, done_ns = srv.wait(tk)
-            lat.append((done_ns - tk.arrival_ns) * 1e-6)
-            ring_give()
+                state["error"] = e
+                lock.notify_all()
+
+    def check_failed():
+        if state.get("error") is not None:
+            raise RuntimeError("serving request failed") from state["error"]
 
     th = threading.Thread(target=completer)
     th.start()
@@ -554,12 +564,14 @@ def bench_serve(args) -> dict:
         else:
             tk = srv.submit(x, ci, co, key=0, result=res, arrival_ns=at)
         with lock:
+            check_failed()
             inflight.append(tk)
             lock.notify_all()
     with lock:
         state["done"] = True
         lock.notify_all()
     th.join()
+    check_failed()
     end_ns = clock_ns()
     st = srv.stats()
     srv.close()

@@ -96,13 +96,26 @@ int dev_alloc(void** p, size_t bytes, hipStream_t s)
     {
         std::lock_guard<std::mutex> lk(P.mu);
         auto& fb = P.free_blocks[tag];
-        auto it = fb.find(sz);
-        if (it != fb.end()) {
+        // exact size class up to 1 MiB; above, the smallest cached block
+        // with at most 1/8 more bytes
+        auto it = sz <= ((size_t)1 << 20) ? fb.find(sz) : fb.lower_bound(sz);
+        if (it != fb.end() && it->first <= sz + sz / 8) {
             *p = it->second;
+            P.live[*p] = LiveBlock{tag, it->first};
             fb.erase(it);
-            P.live[*p] = LiveBlock{tag, sz};
             return RG_OK;
         }
+    }
+    // Keep a headroom of really free device memory (1/64 of the device, at
+    // least 2 GiB) for the runtime's own allocations: below it, the cached
+    // blocks are released before the pool grows.
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess) {
+        size_t headroom = tot / 64;
+        if (headroom < ((size_t)2 << 30)) headroom = (size_t)2 << 30;
+        if (fr < sz + headroom) dev_trim();
+    } else {
+        (void)hipGetLastError();
     }
     void* q = nullptr;
     hipError_t e = hipMalloc(&q, sz);

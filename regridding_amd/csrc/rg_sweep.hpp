@@ -526,6 +526,63 @@ RG_HD double pairwise_sum(const double* a, int64_t n, int64_t stride = 1)
     return ret;
 }
 
+// numpy's split of a pairwise-sum node of n > 128 elements: the left child
+// takes n/2 rounded down to a multiple of 8.
+RG_HD inline int64_t pairwise_split(int64_t n)
+{
+    const int64_t n2 = n / 2;
+    return n2 - n2 % 8;
+}
+
+// The same sum as pairwise_sum, without a stack (no private memory on the
+// GPU): the split tree is walked in post-order, each finished subtree's sum
+// is stored in its first element, and the parent of a node is found again by
+// descending from the root.  Destroys a[0 .. n) (strided).
+RG_HD double pairwise_sum_inplace(double* a, int64_t n, int64_t stride = 1)
+{
+    if (n <= 128) return pairwise_leaf(a, n, stride);
+    int64_t off = 0, len = n;
+    while (len > 128) len = pairwise_split(len);  // leftmost leaf
+    for (;;) {
+        a[off * stride] = pairwise_leaf(a + off * stride, len, stride);
+        for (;;) {
+            if (len == n) return a[0];
+            // parent (po, pl) of node (off, len), and which child it is
+            int64_t po = 0, pl = n;
+            bool left;
+            for (;;) {
+                const int64_t n2 = pairwise_split(pl);
+                if (off == po && len == n2) {
+                    left = true;
+                    break;
+                }
+                if (off == po + n2 && len == pl - n2) {
+                    left = false;
+                    break;
+                }
+                if (off < po + n2) {
+                    pl = n2;
+                } else {
+                    po += n2;
+                    pl -= n2;
+                }
+            }
+            if (left) {
+                // on to the right sibling's leftmost leaf
+                const int64_t n2 = pairwise_split(pl);
+                off = po + n2;
+                len = pl - n2;
+                while (len > 128) len = pairwise_split(len);
+                break;
+            }
+            // both children done: left sum at po, right sum at off
+            a[po * stride] = a[po * stride] + a[off * stride];
+            off = po;
+            len = pl;
+        }
+    }
+}
+
 // np.add.reduceat over one segment: first element plus the pairwise sum of
 // the rest (_weights_arrays.py:72).
 RG_HD double reduceat_segment(const double* a, int64_t n, int64_t st = 1)
@@ -676,7 +733,8 @@ RG_HD int64_t bucket_reduce(uint64_t* keys, double* vals, int64_t n,
 }
 
 // Completion of a bucket left sorted (KV records, weights already divided)
-// by bucket_reduce: any run length.
+// by bucket_reduce: any run length; the values of each run are consumed in
+// place.
 RG_HD int64_t bucket_reduce_long(KV* r, int64_t n)
 {
     int64_t m = 0, p = 0;
@@ -684,7 +742,9 @@ RG_HD int64_t bucket_reduce_long(KV* r, int64_t n)
         uint64_t out = r[p].key >> 32;
         int64_t q = p + 1;
         while (q < n && (r[q].key >> 32) == out) q++;
-        double v = reduceat_segment(&r[p].val, q - p, 2);
+        // reduceat_segment in place: a0 + pairwise(a1..)
+        double* a = &r[p].val;
+        double v = q - p == 1 ? a[0] : a[0] + pairwise_sum_inplace(a + 2, q - p - 1, 2);
         r[m].key = out;
         r[m].val = v;
         m++;

@@ -44,6 +44,8 @@ def lib():
         L.rge_reduceat_segment.restype = ctypes.c_double
         L.rge_reduceat_segment.argtypes = [_pd, _i64]
         L.rge_cell_area.restype = None
+        L.rge_reduceat_segment_inplace.restype = ctypes.c_double
+        L.rge_reduceat_segment_inplace.argtypes = [_pd, _i64, _i64]
         L.rge_cell_area.argtypes = [_pd, _pd, _i64, _i64, _pd]
         L.rge_index_secant.restype = _i64
         L.rge_index_secant.argtypes = [_pd, _pd, _i64, _i64, ctypes.c_double, ctypes.c_double]
@@ -96,6 +98,14 @@ def build_2d(grid_input, grid_output, weights_input=None, K: int = 16, guess: in
 def reduceat_segment(a) -> float:
     a = np.ascontiguousarray(a, dtype=np.float64)
     return float(lib().rge_reduceat_segment(_d(a), a.size))
+
+
+def reduceat_segment_inplace(a, stride: int = 1) -> float:
+    """The stackless in-place variant (on a strided copy of ``a``)."""
+    a = np.asarray(a, dtype=np.float64)
+    buf = np.full(a.size * stride, np.nan)
+    buf[::stride] = a
+    return float(lib().rge_reduceat_segment_inplace(_d(buf), a.size, stride))
 
 
 def cell_area(x, y) -> np.ndarray:

@@ -417,6 +417,11 @@ double rge_reduceat_segment(const double* a, int64_t n)
     return reduceat_segment(a, n);
 }
 
+double rge_reduceat_segment_inplace(double* a, int64_t n, int64_t stride)
+{
+    return n == 1 ? a[0] : a[0] + pairwise_sum_inplace(a + stride, n - 1, stride);
+}
+
 void rge_cell_area(const double* x, const double* y, int64_t nx, int64_t ny,
                    double* out)
 {

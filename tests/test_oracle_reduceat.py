@@ -31,6 +31,17 @@ def test_reduce_segment_matches_numpy(n):
         expected = np.add.reduceat(a, [0])[0]
         assert oracle.reduce_segment(a) == expected
         assert emu.reduceat_segment(a) == expected
+        assert emu.reduceat_segment_inplace(a, 2) == expected
+
+
+@pytest.mark.parametrize("n", [129, 130, 257, 1000, 4099, 20000, 65537, 300001])
+def test_stackless_pairwise_matches_numpy(n):
+    """The in-place post-order walk of the split tree (no stack) that the
+    GPU's long-bucket path uses, against numpy, bit for bit."""
+    rng = np.random.default_rng(n + 1)
+    for stride in (1, 2):
+        a = _hard_values(rng, n)
+        assert emu.reduceat_segment_inplace(a, stride) == np.add.reduceat(a, [0])[0]
 
 
 def test_reduceat_order_is_not_sequential():
