@@ -42,13 +42,18 @@ typedef struct rg_tuning {
     int32_t edges_per_lane; /* sweep edges walked by one lane (default 16) */
     int32_t block;          /* threads per block of the walk kernels (128; a
                                multiple of 64, at most 512) */
-    int32_t flags;          /* RG_TUNE_TIMINGS: record per-phase times */
+    int32_t flags;          /* RG_TUNE_TIMINGS: record per-phase times;
+                               RG_TUNE_GUESS_OUTSIDE: a test hook */
     int32_t stage_per_edge; /* triples staged per sweep edge by the counting
                                walk (default 8); chunks that emit more are
                                walked a second time */
 } rg_tuning;
 
 #define RG_TUNE_TIMINGS 1
+/* Test hook: start every chunk after a row's first from "outside" instead of
+ * the cell-search guess, so the fix-up re-walks most chunks; the result is
+ * the same bits, only slower. */
+#define RG_TUNE_GUESS_OUTSIDE 2
 
 /* Counters and (optionally) phase times of a build. */
 typedef struct rg_build_stats {

@@ -25,11 +25,12 @@ def build_2d_device(
     xo: DeviceArray, yo: DeviceArray, shape_output: tuple[int, int],
     wi: DeviceArray | None = None,
     edges_per_lane: int = 0, block: int = 0, timings: bool = False,
-    stage_per_edge: int = 0, stream: int = 0,
+    stage_per_edge: int = 0, stream: int = 0, flags: int = 0,
 ) -> Weights:
-    """Build from device-resident vertex arrays (row-major float64)."""
-    t = rg_tuning(int(edges_per_lane), int(block), RG_TUNE_TIMINGS if timings else 0,
-                  int(stage_per_edge))
+    """Build from device-resident vertex arrays (row-major float64).
+    ``flags`` adds ``RG_TUNE_*`` bits (test hooks) to the tuning."""
+    t = rg_tuning(int(edges_per_lane), int(block),
+                  (RG_TUNE_TIMINGS if timings else 0) | int(flags), int(stage_per_edge))
     h = ctypes.c_void_p()
     check(
         lib().rg_build_conservative_2d(

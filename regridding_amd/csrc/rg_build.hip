@@ -456,14 +456,16 @@ __device__ inline const Build& build_of_thread(const Build* __restrict__ builds,
 
 __global__ __launch_bounds__(WALK_MAX_BLOCK) void k_chunk_start(const Build* __restrict__ builds,
                               const int32_t* __restrict__ pair_of_wave,
-                              int64_t nthreads_all, int32_t* __restrict__ start)
+                              int64_t nthreads_all, int32_t* __restrict__ start,
+                              int guess_outside)
 {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (t >= nthreads_all) return;
     const Build& B = build_of_thread(builds, pair_of_wave, t);
     const ChunkRef r = chunk_of_thread(B, t - B.thread_gbase);
     if (!r.valid || r.c == 0) return;
-    start[B.chunk_gbase + r.g] = chunk_start_guess(B.pass[r.p], r.row, r.c * B.K);
+    start[B.chunk_gbase + r.g] =
+        guess_outside ? STATE_OUTSIDE : chunk_start_guess(B.pass[r.p], r.row, r.c * B.K);
 }
 
 // ---- walks ---------------------------------------------------------------
@@ -501,7 +503,10 @@ __global__ __launch_bounds__(WALK_MAX_BLOCK) void k_walk_count(const Build* __re
     count[g] = sink.n;
 }
 
-// One thread per row: make every chunk start where its predecessor ended.
+// One wavefront per row: make every chunk start where its predecessor
+// ended.  The lanes check the row's chunk links with coalesced loads; only a
+// row with a mismatched link or a failed chunk is re-walked, by lane 0, in
+// chunk order (a fix can change the end state the next link compares with).
 __global__ __launch_bounds__(WALK_MAX_BLOCK) void k_fixup(const Build* __restrict__ builds,
                         const int32_t* __restrict__ pair_of_row,
                         int64_t nrows_all, int64_t ncells_all,
@@ -511,14 +516,21 @@ __global__ __launch_bounds__(WALK_MAX_BLOCK) void k_fixup(const Build* __restric
                         int32_t* __restrict__ failed,
                         Triple* __restrict__ stage, int64_t cap)
 {
-    const int64_t gr = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (gr >= nrows_all) return;
+    const int64_t gr = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (gr >= nrows_all) return;  // uniform per wavefront
     const Build& B = builds[pair_of_row[gr]];
     const int64_t r = gr - B.row_gbase;
     const int p = pass_of_row(B, r);
     const Pass& P = B.pass[p];
     const int64_t row = r - B.row_base[p];
     const int64_t g0 = B.chunk_gbase + P.chunk_base + row * P.cpr;
+    bool bad = false;
+    for (int64_t c = lane; c < P.cpr; c += 64) {
+        if (end[g0 + c] == STATE_FAILED) bad = true;
+        if (c >= 1 && start[g0 + c] != end[g0 + c - 1]) bad = true;
+    }
+    if (__ballot(bad) == 0 || lane != 0) return;
     int32_t* hist = hist_all + B.cell_gbase;
     int32_t fixes = 0;
     for (int64_t c = 1; c < P.cpr; c++) {
@@ -1101,11 +1113,13 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
     int64_t K = 16, stage_per_edge = 8;
     int block = 128;
     bool timings = false;
+    int32_t flags = 0;
     if (tuning) {
         if (tuning->edges_per_lane > 0) K = tuning->edges_per_lane;
         if (tuning->block > 0) block = tuning->block;
         if (tuning->stage_per_edge > 0) stage_per_edge = tuning->stage_per_edge;
-        timings = (tuning->flags & RG_TUNE_TIMINGS) != 0;
+        flags = tuning->flags;
+        timings = (flags & RG_TUNE_TIMINGS) != 0;
     }
     if (block % 64 != 0 || block > WALK_MAX_BLOCK)
         return fail(RG_ERR_ARGUMENT, "block must be a multiple of 64, <= 512");
@@ -1330,8 +1344,8 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
     k_row_start<<<grid_for(row_tot * 64, 256), 256, 0, s>>>(
         d_builds, pair_of_row.p, row_tot, start.p, counters.p + 0);
     RG_HIP(hipGetLastError());
-    k_chunk_start<<<grid_for(thr_tot, block), block, 0, s>>>(d_builds, pair_of_wave.p,
-                                                           thr_tot, start.p);
+    k_chunk_start<<<grid_for(thr_tot, block), block, 0, s>>>(
+        d_builds, pair_of_wave.p, thr_tot, start.p, (flags & RG_TUNE_GUESS_OUTSIDE) != 0);
     RG_HIP(hipGetLastError());
     tm.mark();  // 2
 
@@ -1346,7 +1360,7 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
     tm.mark();  // 3
 
     // ---- fix-up
-    k_fixup<<<grid_for(row_tot, 64), 64, 0, s>>>(
+    k_fixup<<<grid_for(row_tot * 64, 256), 256, 0, s>>>(
         d_builds, pair_of_row.p, row_tot, cell_tot, start.p, end.p, count.p, hist.p,
         counters.p + 1, counters.p + 2, stage.p, cap);
     RG_HIP(hipGetLastError());

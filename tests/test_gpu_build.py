@@ -78,6 +78,20 @@ def test_build_staging_overflow(stage_per_edge):
         assert h.stats()["n_restaged"] > 0
 
 
+@pytest.mark.parametrize("case", ["c1", "coarse_to_fine", "very_coarse_to_fine"])
+@pytest.mark.parametrize("K", [4, 32])
+def test_build_fixup_rewalks_bad_guesses(case, K):
+    """Every chunk after a row's first starts from "outside" (a test hook):
+    the fix-up re-walks each mislinked chunk (un-counting and re-staging its
+    triples) and the weights are still the oracle's bits."""
+    ci, co = _perturbed(*CASES[case]())
+    h = build_2d(ci, co, edges_per_lane=K, flags=2)
+    got = h.export()
+    for a, b in zip(got, oracle.weights_2d(ci, co)):
+        assert np.array_equal(a, b)
+    assert h.stats()["n_fixups"] > 0
+
+
 def test_build_weights_input():
     ci, co = _perturbed(*CASES["coarse_to_fine"]())
     wi = np.random.default_rng(7).uniform(0.5, 3.0, (ci[0].shape[0] - 1, ci[0].shape[1] - 1))
