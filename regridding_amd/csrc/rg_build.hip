@@ -35,11 +35,14 @@ namespace rg {
 
 // ---- atomics usable from the shared walk code ----------------------------
 
-// Input-cell buckets are split by sweep pass and laid out pass-major:
-// bucket (pass p, cell c) at slot p * C + c (C cells in all).  The triples
-// one pass writes for neighbouring cells -- which neighbouring lanes emit at
-// about the same time -- then share cache lines, both for the histogram
-// atomics and for the scattered records.
+// Triples are bucketed by sweep pass and by group of 64 consecutive input
+// cells (every pair's cell space is padded to a multiple of 64, so a group
+// belongs to one pair), pass-major: bucket (pass p, group g) at slot
+// p * G + g (G groups in all).  The histogram of buckets is small enough to
+// stay in cache, a walk's consecutive triples mostly fall in one bucket, and
+// the scatter claims whole runs of records per bucket.  Each record carries
+// its cell within the group in a side array of bytes; the reduction splits
+// the group by cell.
 
 // Histogram increments are combined in registers: a walk's consecutive
 // triples mostly fall in one or two (cell, pass) buckets (the static cell of
@@ -80,7 +83,7 @@ struct StageSink {
     Triple* __restrict__ stage;
     int32_t* __restrict__ hist;
     int32_t p;
-    int64_t ps;  // bucket (pass p, cell c) is hist[p * ps + c]
+    int64_t ps;  // bucket (pass p, cell c) is hist[p * ps + (c >> 6)]
     HistCache hc;
     __device__ void emit(int64_t cin, int64_t cout, double a)
     {
@@ -92,7 +95,7 @@ struct StageSink {
             stage[stage_index(g, n, cap)] = t;
         }
         n++;
-        hc.add(hist, p * ps + cin, 1);
+        hc.add(hist, p * ps + (cin >> 6), 1);
     }
     __device__ void flush() { hc.flush(hist); }
 };
@@ -104,46 +107,46 @@ struct UncountSink {
     HistCache hc;
     __device__ void emit(int64_t cin, int64_t, double)
     {
-        hc.add(hist, p * ps + cin, -1);
+        hc.add(hist, p * ps + (cin >> 6), -1);
     }
     __device__ void flush() { hc.flush(hist); }
 };
 
 // Bucket cursors: cursor[b] starts at the first record of bucket b and each
-// store claims the next one with one returning atomic (record positions fit
-// in 32 bits: a build holds fewer than 2^32 triples).  A store is only
+// store claims records with a returning atomic (record positions fit in 32
+// bits: a build holds fewer than 2^32 triples).  A store is only
 // bounds-checked against the record count; k_cursor_check then verifies that
 // every cursor ended exactly at its bucket's end, which holds iff each
 // bucket received exactly the triples the count walk histogrammed.
-__device__ inline void put_records(uint32_t* __restrict__ cursor,
-                                   KV* __restrict__ rec, uint32_t n_rec,
-                                   int32_t* __restrict__ overflow, int64_t b,
-                                   int64_t cout, uint32_t ordinal, double a)
+__device__ inline void put_record(KV* __restrict__ rec, uint8_t* __restrict__ rcell,
+                                  uint32_t pos, int64_t cin, int64_t cout, uint32_t ordinal,
+                                  double a)
 {
-    const uint32_t pos = atomicAdd(&cursor[b], 1u);
-    if (pos < n_rec) {
-        KV r;
-        r.key = ((uint64_t)(uint32_t)cout << 32) | (uint64_t)ordinal;
-        r.val = a;
-        rec[pos] = r;
-    } else {
-        atomicAdd(overflow, 1);
-    }
+    KV r;
+    r.key = ((uint64_t)(uint32_t)cout << 32) | (uint64_t)ordinal;
+    r.val = a;
+    rec[pos] = r;
+    rcell[pos] = (uint8_t)(cin & 63);
 }
 
-// Writes each triple into its input cell's bucket (chunks whose triples
-// overflowed the staging buffer walk again with this sink).
+// Writes each triple into its bucket with its own claim (chunks whose
+// triples overflowed the staging buffer walk again with this sink).
 struct BucketSink {
     uint32_t ordinal;
     uint32_t* __restrict__ cursor;
     KV* __restrict__ rec;
+    uint8_t* __restrict__ rcell;
     uint32_t n_rec;
     int32_t* __restrict__ overflow;
     int32_t p;
     int64_t ps;
     __device__ void emit(int64_t cin, int64_t cout, double a)
     {
-        put_records(cursor, rec, n_rec, overflow, p * ps + cin, cout, ordinal, a);
+        const uint32_t pos = atomicAdd(&cursor[p * ps + (cin >> 6)], 1u);
+        if (pos < n_rec)
+            put_record(rec, rcell, pos, cin, cout, ordinal, a);
+        else
+            atomicAdd(overflow, 1);
         ordinal++;
     }
 };
@@ -481,7 +484,7 @@ __device__ inline void chunk_range(const Build& B, const Pass& P, int64_t c,
 
 __global__ __launch_bounds__(WALK_MAX_BLOCK) void k_walk_count(const Build* __restrict__ builds,
                              const int32_t* __restrict__ pair_of_wave,
-                             int64_t nthreads_all, int64_t ncells_all,
+                             int64_t nthreads_all, int64_t ngroups_all,
                              const int32_t* __restrict__ start,
                              int32_t* __restrict__ end,
                              int32_t* __restrict__ count,
@@ -497,7 +500,7 @@ __global__ __launch_bounds__(WALK_MAX_BLOCK) void k_walk_count(const Build* __re
     const int64_t g = B.chunk_gbase + r.g;
     int64_t k0, k1;
     chunk_range(B, P, r.c, &k0, &k1);
-    StageSink sink{0, t, cap, stage, hist + B.cell_gbase, r.p, ncells_all, {}};
+    StageSink sink{0, t, cap, stage, hist + (B.cell_gbase >> 6), r.p, ngroups_all, {}};
     end[g] = walk_chunk(B, P, r.row, k0, k1, start[g], sink);
     sink.flush();
     count[g] = sink.n;
@@ -509,7 +512,7 @@ __global__ __launch_bounds__(WALK_MAX_BLOCK) void k_walk_count(const Build* __re
 // chunk order (a fix can change the end state the next link compares with).
 __global__ __launch_bounds__(WALK_MAX_BLOCK) void k_fixup(const Build* __restrict__ builds,
                         const int32_t* __restrict__ pair_of_row,
-                        int64_t nrows_all, int64_t ncells_all,
+                        int64_t nrows_all, int64_t ngroups_all,
                         int32_t* __restrict__ start,
                         int32_t* __restrict__ end, int32_t* __restrict__ count,
                         int32_t* __restrict__ hist_all, int32_t* __restrict__ nfix,
@@ -531,7 +534,7 @@ __global__ __launch_bounds__(WALK_MAX_BLOCK) void k_fixup(const Build* __restric
         if (c >= 1 && start[g0 + c] != end[g0 + c - 1]) bad = true;
     }
     if (__ballot(bad) == 0 || lane != 0) return;
-    int32_t* hist = hist_all + B.cell_gbase;
+    int32_t* hist = hist_all + (B.cell_gbase >> 6);
     int32_t fixes = 0;
     for (int64_t c = 1; c < P.cpr; c++) {
         const int64_t g = g0 + c;
@@ -540,11 +543,11 @@ __global__ __launch_bounds__(WALK_MAX_BLOCK) void k_fixup(const Build* __restric
         if (start[g] == truth) continue;
         int64_t k0, k1;
         chunk_range(B, P, c, &k0, &k1);
-        UncountSink undo{hist, p, ncells_all, {}};
+        UncountSink undo{hist, p, ngroups_all, {}};
         (void)walk_chunk(B, P, row, k0, k1, start[g], undo);
         undo.flush();
         StageSink redo{0, B.thread_gbase + thread_of_chunk(P, row, c), cap, stage,
-                       hist, p, ncells_all, {}};
+                       hist, p, ngroups_all, {}};
         end[g] = walk_chunk(B, P, row, k0, k1, truth, redo);
         redo.flush();
         count[g] = redo.n;
@@ -559,108 +562,7 @@ __global__ __launch_bounds__(WALK_MAX_BLOCK) void k_fixup(const Build* __restric
     if (fixes) atomicAdd(nfix, fixes);
 }
 
-// Staged triples -> input-cell buckets, each tagged with its emission
-// ordinal (the chunk's first ordinal plus its position in the chunk).
-// Chunks whose triples overflowed the staging walk again, emitting straight
-// into the buckets.
-__global__ __launch_bounds__(WALK_MAX_BLOCK) void k_scatter(const Build* __restrict__ builds,
-                          const int32_t* __restrict__ pair_of_wave,
-                          int64_t nthreads_all, int64_t ncells_all,
-                          const int32_t* __restrict__ start,
-                          const int32_t* __restrict__ count,
-                          const Triple* __restrict__ stage, int64_t cap,
-                          const int64_t* __restrict__ chunk_off,
-                          uint32_t* __restrict__ cursor_all,
-                          KV* __restrict__ rec, uint32_t n_rec,
-                          int32_t* __restrict__ overflow,
-                          int32_t* __restrict__ restaged)
-{
-    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (t >= nthreads_all) return;
-    const Build& B = build_of_thread(builds, pair_of_wave, t);
-    const ChunkRef r = chunk_of_thread(B, t - B.thread_gbase);
-    if (!r.valid) return;
-    const int64_t g = B.chunk_gbase + r.g;
-    uint32_t* cursor = cursor_all + B.cell_gbase;
-    const int32_t n = count[g];
-    const uint32_t ord = (uint32_t)chunk_off[g];
-    if (n > cap) {
-        const Pass& P = B.pass[r.p];
-        int64_t k0, k1;
-        chunk_range(B, P, r.c, &k0, &k1);
-        BucketSink sink{ord, cursor, rec, n_rec, overflow, r.p, ncells_all};
-        (void)walk_chunk(B, P, r.row, k0, k1, start[g], sink);
-        atomicAdd(restaged, 1);
-        return;
-    }
-    // consecutive staged triples of one bucket (a walk inside one static
-    // cell of a sweep over the output grid) claim their records with one
-    // atomic
-    int32_t q0 = 0;
-    while (q0 < n) {
-        const Triple e0 = stage[stage_index(t, q0, cap)];
-        int32_t q1 = q0 + 1;
-        while (q1 < n && q1 - q0 < 16 && stage[stage_index(t, q1, cap)].cin == e0.cin)
-            q1++;
-        const uint32_t m = (uint32_t)(q1 - q0);
-        const uint32_t pos = atomicAdd(&cursor[r.p * ncells_all + e0.cin], m);
-        if ((uint64_t)pos + m <= n_rec) {
-            for (int32_t q = q0; q < q1; q++) {
-                const Triple e = q == q0 ? e0 : stage[stage_index(t, q, cap)];
-                KV v;
-                v.key = ((uint64_t)(uint32_t)e.cout << 32) | (uint64_t)(ord + (uint32_t)q);
-                v.val = e.a;
-                rec[pos + (uint32_t)(q - q0)] = v;
-            }
-        } else {
-            atomicAdd(overflow, 1);
-        }
-        q0 = q1;
-    }
-}
-
-// ---- reduce and CSR ------------------------------------------------------
-
-// Coalescing, one wavefront per group of 64 consecutive input cells.  The
-// group's records sit in four pass ranges (the buckets are pass-major); each
-// lane copies its cell's four pieces into a cell-major LDS layout.  Each
-// lane divides its cell's terms by the cell area, sorts them and reduces
-// equal outputs (bucket_reduce); the wave writes the group's unique
-// (out, weight) pairs packed, in row order, into the cell-major output
-// `red` from the group's first cell offset.  Groups beyond the LDS share go
-// through it in runs of consecutive cells, each cell reduced in place at
-// its own cell offset (the group is then marked unpacked); a single cell
-// beyond the share is left to k_bucket_long.
-constexpr int RED_WAVES = 2;     // wavefronts per block
-constexpr int RED_CAP = 1024;    // LDS entries per wavefront (16 KiB)
-
-// pass-range position -> LDS slot, for the cells of one wavefront's group
-struct RedTables {
-    int32_t offp[4][65];  // per pass: cell l's first position in the range
-    int32_t tab[4][64];   // per pass: LDS slot of position q is q + tab[p][l]
-};
-
-// Load the records of cells [c0, c1) (group-local) into LDS slots
-// (tab[p][l] + q - shift): each lane copies its own cell's piece of every
-// pass range (the pieces of neighbouring lanes are neighbouring in memory).
-__device__ inline void load_cells(const KV* __restrict__ rec, const int64_t* ps,
-                                  const RedTables& T, int c0, int c1, int32_t shift,
-                                  uint64_t* K, double* V, int lane)
-{
-    if (lane < c0 || lane >= c1) return;
-#pragma unroll
-    for (int p = 0; p < 4; p++) {
-        const int32_t q0 = T.offp[p][lane], q1 = T.offp[p][lane + 1];
-        const KV* src = rec + ps[p];
-        const int32_t d = T.tab[p][lane] - shift;
-        for (int32_t q = q0; q < q1; q++) {
-            const KV r = src[q];
-            K[q + d] = r.key;
-            V[q + d] = r.val;
-        }
-    }
-}
-
+// Orders the LDS accesses of a wavefront's lanes before and after it.
 __device__ inline void wave_sync_lds()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -668,72 +570,249 @@ __device__ inline void wave_sync_lds()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Staged triples -> buckets, each tagged with its emission ordinal (the
+// chunk's first ordinal plus its position in the chunk) and its cell byte.
+// A wavefront takes its lanes' triples SCAT_R per lane per round and groups
+// them by bucket in an LDS hash table (one entry per bucket met in the
+// round: bucket, count, claimed base); one lane per entry claims the
+// bucket's records with a single atomic, and the wave writes every
+// bucket's records as one contiguous run.  A full table falls back to one
+// claim per triple.  Chunks whose triples overflowed the staging walk again,
+// emitting straight into the buckets.
+constexpr int SCAT_R = 8;     // triples per lane per round
+constexpr int SCAT_HT = 256;  // hash entries per wavefront
+
+struct ScatTable {
+    int32_t key[SCAT_HT];
+    int32_t cnt[SCAT_HT];
+    uint32_t base[SCAT_HT];
+};
+
+__global__ __launch_bounds__(WALK_MAX_BLOCK) void k_scatter(const Build* __restrict__ builds,
+                          const int32_t* __restrict__ pair_of_wave,
+                          int64_t nthreads_all, int64_t ngroups_all,
+                          const int32_t* __restrict__ start,
+                          const int32_t* __restrict__ count,
+                          const Triple* __restrict__ stage, int64_t cap,
+                          const int64_t* __restrict__ chunk_off,
+                          uint32_t* __restrict__ cursor_all,
+                          KV* __restrict__ rec, uint8_t* __restrict__ rcell,
+                          uint32_t n_rec,
+                          int32_t* __restrict__ overflow,
+                          int32_t* __restrict__ restaged)
+{
+    __shared__ ScatTable tables[WALK_MAX_BLOCK / 64];
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= nthreads_all) return;  // nthreads_all is a multiple of 64: whole waves
+    const int lane = threadIdx.x & 63;
+    ScatTable& H = tables[threadIdx.x >> 6];
+    const Build& B = build_of_thread(builds, pair_of_wave, t);
+    const ChunkRef r = chunk_of_thread(B, t - B.thread_gbase);
+    const int64_t g = B.chunk_gbase + r.g;
+    // every lane of the wave sweeps one pass: the bucket row is wave-uniform
+    uint32_t* cursor = cursor_all + (B.cell_gbase >> 6) + (int64_t)r.p * ngroups_all;
+    const int32_t n = r.valid ? count[g] : 0;
+    const uint32_t ord = r.valid ? (uint32_t)chunk_off[g] : 0;
+    const int32_t nst = n > cap ? 0 : n;  // staged triples this lane scatters
+    int32_t mx = nst;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
+    for (int h = lane; h < SCAT_HT; h += 64) {
+        H.key[h] = -1;
+        H.cnt[h] = 0;
+    }
+    wave_sync_lds();
+    for (int32_t q0 = 0; q0 < mx; q0 += SCAT_R) {
+        Triple e[SCAT_R];
+        int32_t slot[SCAT_R], local[SCAT_R];
+#pragma unroll
+        for (int k = 0; k < SCAT_R; k++) {
+            slot[k] = -2;  // no triple
+            if (q0 + k < nst) {
+                e[k] = stage[stage_index(t, q0 + k, cap)];
+                const int32_t b = e[k].cin >> 6;
+                int32_t h = b & (SCAT_HT - 1);
+                slot[k] = -1;  // table full: claim alone
+                for (int probe = 0; probe < SCAT_HT; probe++) {
+                    const int32_t cur = atomicCAS(&H.key[h], -1, b);
+                    if (cur == -1 || cur == b) {
+                        slot[k] = h;
+                        break;
+                    }
+                    h = (h + 1) & (SCAT_HT - 1);
+                }
+                if (slot[k] >= 0) local[k] = atomicAdd(&H.cnt[h], 1);
+            }
+        }
+        wave_sync_lds();
+        for (int h = lane; h < SCAT_HT; h += 64)
+            if (H.key[h] >= 0) H.base[h] = atomicAdd(&cursor[H.key[h]], (uint32_t)H.cnt[h]);
+        wave_sync_lds();
+#pragma unroll
+        for (int k = 0; k < SCAT_R; k++) {
+            if (slot[k] == -2) continue;
+            const uint32_t pos = slot[k] >= 0
+                                     ? H.base[slot[k]] + (uint32_t)local[k]
+                                     : atomicAdd(&cursor[e[k].cin >> 6], 1u);
+            if (pos < n_rec)
+                put_record(rec, rcell, pos, e[k].cin, e[k].cout, ord + (uint32_t)(q0 + k),
+                           e[k].a);
+            else
+                atomicAdd(overflow, 1);
+        }
+        wave_sync_lds();
+        for (int h = lane; h < SCAT_HT; h += 64) {
+            H.key[h] = -1;
+            H.cnt[h] = 0;
+        }
+        wave_sync_lds();
+    }
+    if (r.valid && n > cap) {
+        const Pass& P = B.pass[r.p];
+        int64_t k0, k1;
+        chunk_range(B, P, r.c, &k0, &k1);
+        BucketSink sink{ord, cursor_all + (B.cell_gbase >> 6), rec, rcell, n_rec, overflow,
+                        r.p, ngroups_all};
+        (void)walk_chunk(B, P, r.row, k0, k1, start[g], sink);
+        atomicAdd(restaged, 1);
+    }
+}
+
+// ---- reduce and CSR ------------------------------------------------------
+
+// Coalescing, one wavefront per group of 64 consecutive input cells.  The
+// group's records sit in four pass ranges (the buckets are pass-major), in
+// claim order, each with its cell byte.  The wave counts them per cell in
+// LDS (the cell-major offsets of the group's output, written to cell_off),
+// places them cell-major in LDS, and each lane divides its cell's terms by
+// the cell area, sorts them and reduces equal outputs (bucket_reduce); the
+// wave writes the group's unique (out, weight) pairs packed, in row order,
+// from the group's first offset.  A group beyond the LDS share is first
+// copied cell-major into `red` (global), then goes through LDS in runs of
+// consecutive cells, each cell reduced in place at its own offset (the
+// group is then marked unpacked); a single cell beyond the share is left to
+// k_bucket_long.
+constexpr int RED_WAVES = 2;     // wavefronts per block
+constexpr int RED_CAP = 1024;    // LDS entries per wavefront (16 KiB)
+
+// Records of one group (all passes): the group totals that give the
+// group-major output offsets.
+__global__ void k_group_counts(const int64_t* __restrict__ bucket_off, int64_t n_groups,
+                               int32_t* __restrict__ cnt)
+{
+    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (g >= n_groups) return;
+    int64_t t = 0;
+#pragma unroll
+    for (int p = 0; p < 4; p++)
+        t += bucket_off[p * n_groups + g + 1] - bucket_off[p * n_groups + g];
+    cnt[g] = (int32_t)t;
+}
+
 __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
-    const int64_t* __restrict__ bucket_off, int64_t n_in,
-    const int64_t* __restrict__ cell_off, const KV* __restrict__ rec,
-    KV* __restrict__ red, const double* __restrict__ vol_in,
+    const int64_t* __restrict__ bucket_off, int64_t n_groups,
+    const int64_t* __restrict__ group_off, const KV* __restrict__ rec,
+    const uint8_t* __restrict__ rcell, KV* __restrict__ red,
+    int64_t* __restrict__ cell_off, const double* __restrict__ vol_in,
     const double* __restrict__ wi, int32_t* __restrict__ nnz_row,
     int32_t* __restrict__ group_mode, int32_t* __restrict__ long_list,
     int32_t* __restrict__ n_long)
 {
     __shared__ uint64_t sk[RED_WAVES][RED_CAP];
     __shared__ double sv[RED_WAVES][RED_CAP];
-    __shared__ RedTables tabs[RED_WAVES];
+    __shared__ int32_t scnt[RED_WAVES][64];
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t grp = (int64_t)blockIdx.x * RED_WAVES + wid;
-    const int64_t i0 = grp * 64;
-    if (i0 >= n_in) return;  // uniform per wavefront; no block barriers below
-    const int64_t i1 = i0 + 64 < n_in ? i0 + 64 : n_in;
-    const int nl = (int)(i1 - i0);
-    const int64_t i = i0 + lane;
-    const int64_t base = cell_off[i0];
-    const int64_t total = cell_off[i1] - base;
-    RedTables& T = tabs[wid];
-    int64_t ps[4];
-    int64_t s = 0, n = 0;
-    double vol = 1;
-    if (i < i1) {
-        s = cell_off[i] - base;
-        n = cell_off[i + 1] - cell_off[i];
-        vol = vol_in[i];
-    }
-    int32_t pre = 0;
+    if (grp >= n_groups) return;  // uniform per wavefront; no block barriers below
+    const int64_t i = grp * 64 + lane;
+    const int64_t base = group_off[grp];
+    const int64_t total = group_off[grp + 1] - base;
+    int32_t* C = scnt[wid];
+    // the group's pass ranges [ps_p, ps_p + len_p), concatenated: record q
+    // of the group sits at src_of(q)
+    const int64_t ps0 = bucket_off[grp], ps1 = bucket_off[n_groups + grp];
+    const int64_t ps2 = bucket_off[2 * n_groups + grp], ps3 = bucket_off[3 * n_groups + grp];
+    const int64_t e0 = bucket_off[grp + 1] - ps0;
+    const int64_t e1 = e0 + bucket_off[n_groups + grp + 1] - ps1;
+    const int64_t e2 = e1 + bucket_off[2 * n_groups + grp + 1] - ps2;
+    auto src_of = [=](int64_t q) -> int64_t {
+        return q < e0 ? ps0 + q : q < e1 ? ps1 + (q - e0) : q < e2 ? ps2 + (q - e1)
+                                                               : ps3 + (q - e2);
+    };
+    const bool fits = total <= RED_CAP;
+    // a group that fits: its records are read once, into registers (record
+    // q = lane + 64 u of the concatenated pass ranges)
+    KV rr[RED_CAP / 64];
+    int32_t cc[RED_CAP / 64];
+    C[lane] = 0;
+    wave_sync_lds();
+    if (fits) {
 #pragma unroll
-    for (int p = 0; p < 4; p++) {
-        ps[p] = bucket_off[p * n_in + i0];
-        if (i < i1) {
-            const int64_t b = bucket_off[p * n_in + i];
-            const int32_t o = (int32_t)(b - ps[p]);
-            T.offp[p][lane] = o;
-            T.tab[p][lane] = (int32_t)s + pre - o;
-            pre += (int32_t)(bucket_off[p * n_in + i + 1] - b);
+        for (int u = 0; u < RED_CAP / 64; u++) {
+            const int64_t q = lane + 64 * u;
+            cc[u] = -1;
+            if (q < total) {
+                const int64_t src = src_of(q);
+                cc[u] = rcell[src];
+                rr[u] = rec[src];
+            }
         }
-        if (lane == 0) T.offp[p][nl] = (int32_t)(bucket_off[p * n_in + i1] - ps[p]);
+#pragma unroll
+        for (int u = 0; u < RED_CAP / 64; u++)
+            if (cc[u] >= 0) atomicAdd(&C[cc[u]], 1);
+    } else {
+        for (int64_t q0 = lane; q0 < total; q0 += 256) {
+            int32_t c4[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                c4[u] = q0 + 64 * u < total ? rcell[src_of(q0 + 64 * u)] : -1;
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (c4[u] >= 0) atomicAdd(&C[c4[u]], 1);
+        }
     }
+    wave_sync_lds();
+    // records per cell, and the cell-major offsets
+    const int32_t n = C[lane];
+    int32_t incl = n;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int32_t y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    const int32_t s = incl - n;
+    cell_off[i] = base + s;
+    if (grp == n_groups - 1 && lane == 63) cell_off[i + 1] = base + total;
+    const double vol = vol_in[i];
+    wave_sync_lds();
+    C[lane] = s;  // placement cursors
     wave_sync_lds();
     uint64_t* K = sk[wid];
     double* V = sv[wid];
-    bool lr = false;
-    if (total <= RED_CAP) {
-        load_cells(rec, ps, T, 0, nl, 0, K, V, lane);
+    if (fits) {
+#pragma unroll
+        for (int u = 0; u < RED_CAP / 64; u++)
+            if (cc[u] >= 0) {
+                const int32_t d = atomicAdd(&C[cc[u]], 1);
+                K[d] = rr[u].key;
+                V[d] = rr[u].val;
+            }
         wave_sync_lds();
-        int64_t m = 0;
-        if (i < i1) {
-            for (int64_t q = 0; q < n; q++) V[s + q] = weight_of(V[s + q], vol, wi, i);
-            m = bucket_reduce(K + s, V + s, n, &lr);
-        }
+        bool lr = false;
+        for (int32_t q = 0; q < n; q++) V[s + q] = weight_of(V[s + q], vol, wi, i);
+        int64_t m = bucket_reduce(K + s, V + s, n, &lr);
         if (lr) m = n;  // left sorted, uncompacted, for k_bucket_long
         const bool packed = __ballot(lr) == 0;
         int64_t dst = s;  // a group with a long run keeps cell starts
         if (packed) {
             // exclusive prefix of m over the wave: packed destination
-            int64_t incl = m;
+            int64_t mi = m;
 #pragma unroll
             for (int o = 1; o < 64; o <<= 1) {
-                int64_t y = __shfl_up(incl, o, 64);
-                if (lane >= o) incl += y;
+                const int64_t y = __shfl_up(mi, o, 64);
+                if (lane >= o) mi += y;
             }
-            dst = incl - m;
+            dst = mi - m;
         }
         for (int64_t t = 0; t < m; t++) {
             KV r;
@@ -741,31 +820,49 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
             r.val = V[s + t];
             red[base + dst + t] = r;
         }
-        if (i < i1) {
-            nnz_row[i] = (int32_t)m;
-            if (lr) long_list[atomicAdd(n_long, 1)] = (int32_t)i;
-        }
+        nnz_row[i] = (int32_t)m;
+        if (lr) long_list[atomicAdd(n_long, 1)] = (int32_t)i;
         if (lane == 0) group_mode[grp] = packed ? 0 : 1;
     } else {
-        int c = 0;  // group-local first cell of the next run
-        while (c < nl) {
-            const int64_t sb = cell_off[i0 + c] - base;  // wave-uniform
-            const bool in = lane >= c && i < i1 && s + n - sb <= RED_CAP;
+        // cell-major copy of the group in `red`, read back by the wave's lanes
+        for (int64_t q0 = lane; q0 < total; q0 += 128) {
+            // two records in flight per lane
+            const int64_t qa = q0, qb = q0 + 64;
+            const int64_t sa = src_of(qa), sb2 = qb < total ? src_of(qb) : sa;
+            const int32_t ca = rcell[sa];
+            const KV ra = rec[sa];
+            const int32_t cb = rcell[sb2];
+            const KV rb = rec[sb2];
+            red[base + atomicAdd(&C[ca], 1)] = ra;
+            if (qb < total) red[base + atomicAdd(&C[cb], 1)] = rb;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        int c = 0;  // first cell of the next run
+        while (c < 64) {
+            const int32_t sb = __shfl(s, c, 64);  // wave-uniform
+            const bool in = lane >= c && s + n - sb <= RED_CAP;
             const uint64_t mask = __ballot(in);
             if (mask == 0) {
-                // a cell beyond the LDS share: gathered and finished by
-                // k_bucket_long (flagged in the long list)
+                // a cell beyond the LDS share: finished by k_bucket_long
+                // (flagged in the long list)
                 if (lane == c) long_list[atomicAdd(n_long, 1)] = (int32_t)i | INT32_MIN;
                 c++;
                 continue;
             }
             const int d = c + __popcll(mask);  // cells [c, d) fit
-            load_cells(rec, ps, T, c, d, (int32_t)sb, K, V, lane);
+            const int32_t se = __shfl(s + n, d - 1, 64);
+            for (int32_t q = lane; q < se - sb; q += 64) {
+                const KV r = red[base + sb + q];
+                K[q] = r.key;
+                V[q] = r.val;
+            }
             wave_sync_lds();
             if (in) {
                 uint64_t* Kc = K + (s - sb);
                 double* Vc = V + (s - sb);
-                for (int64_t q = 0; q < n; q++) Vc[q] = weight_of(Vc[q], vol, wi, i);
+                for (int32_t q = 0; q < n; q++) Vc[q] = weight_of(Vc[q], vol, wi, i);
                 bool lrc = false;
                 int64_t m = bucket_reduce(Kc, Vc, n, &lrc);
                 if (lrc) m = n;  // left sorted, uncompacted, for k_bucket_long
@@ -785,27 +882,11 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
     }
 }
 
-// Records of one input cell (all passes): the per-cell counts that give
-// the cell-major output offsets.
-__global__ void k_cell_counts(const int64_t* __restrict__ bucket_off, int64_t n_in,
-                              int32_t* __restrict__ cnt)
-{
-    const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (c >= n_in) return;
-    int64_t t = 0;
-#pragma unroll
-    for (int p = 0; p < 4; p++) t += bucket_off[p * n_in + c + 1] - bucket_off[p * n_in + c];
-    cnt[c] = (int32_t)t;
-}
-
-// Cells with a run of more than 129 terms (see bucket_reduce), left sorted
-// at their cell offset by k_reduce_groups, and cells beyond its LDS share
-// (flagged with the sign bit: gathered from their four pass pieces, divided
-// and sorted here first), finished in global memory.  Grid-stride over the
-// normally empty list; fixes nnz_row.
-__global__ void k_bucket_long(const int64_t* __restrict__ bucket_off, int64_t n_in,
-                              const int64_t* __restrict__ cell_off,
-                              const KV* __restrict__ rec, KV* __restrict__ red,
+// Cells the group reduction left over: a cell beyond the LDS share (flag
+// bit set; its records already sit cell-major in `red`) is weighted,
+// sorted and reduced here; a cell with a run longer than the LDS reduction
+// takes (left sorted and weighted in `red`) is reduced here.
+__global__ void k_bucket_long(const int64_t* __restrict__ cell_off, KV* __restrict__ red,
                               const double* __restrict__ vol_in,
                               const double* __restrict__ wi,
                               const int32_t* __restrict__ long_list,
@@ -820,12 +901,6 @@ __global__ void k_bucket_long(const int64_t* __restrict__ bucket_off, int64_t n_
         const int64_t start = cell_off[i], n = cell_off[i + 1] - start;
         KV* r = red + start;
         if (e < 0) {
-            int64_t k = 0;
-            for (int p = 0; p < 4; p++) {
-                const int64_t b0 = bucket_off[p * n_in + i];
-                const int64_t b1 = bucket_off[p * n_in + i + 1];
-                for (int64_t t = b0; t < b1; t++) r[k++] = rec[t];
-            }
             const double vol = vol_in[i];
             for (int64_t t = 0; t < n; t++) r[t].val = weight_of(r[t].val, vol, wi, i);
             sort_records(r, n);
@@ -1332,14 +1407,15 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
     tm.mark();  // 1
 
     // ---- start states
+    const int64_t n_groups = cell_tot / 64;  // every pair's cells padded to 64
     DevBuf<int32_t> start, end, count, hist, counters;
     RG_TRY(start.alloc(chunk_tot, s));
     RG_TRY(end.alloc(chunk_tot, s));
     RG_TRY(count.alloc(chunk_tot, s));
-    RG_TRY(hist.alloc(4 * cell_tot, s));
+    RG_TRY(hist.alloc(4 * n_groups, s));
     // anomalies, fixups, failed/overflow, long runs, restaged chunks
     RG_TRY(counters.alloc(8, s));
-    RG_TRY(fill_zero(hist.p, 4 * cell_tot, s));
+    RG_TRY(fill_zero(hist.p, 4 * n_groups, s));
     RG_TRY(fill_zero(counters.p, 8, s));
     k_row_start<<<grid_for(row_tot * 64, 256), 256, 0, s>>>(
         d_builds, pair_of_row.p, row_tot, start.p, counters.p + 0);
@@ -1354,25 +1430,25 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
     DevBuf<Triple> stage;
     RG_TRY(stage.alloc((size_t)thr_tot * cap, s));
     k_walk_count<<<grid_for(thr_tot, block), block, 0, s>>>(
-        d_builds, pair_of_wave.p, thr_tot, cell_tot, start.p, end.p, count.p, hist.p,
+        d_builds, pair_of_wave.p, thr_tot, n_groups, start.p, end.p, count.p, hist.p,
         stage.p, cap);
     RG_HIP(hipGetLastError());
     tm.mark();  // 3
 
     // ---- fix-up
     k_fixup<<<grid_for(row_tot * 64, 256), 256, 0, s>>>(
-        d_builds, pair_of_row.p, row_tot, cell_tot, start.p, end.p, count.p, hist.p,
+        d_builds, pair_of_row.p, row_tot, n_groups, start.p, end.p, count.p, hist.p,
         counters.p + 1, counters.p + 2, stage.p, cap);
     RG_HIP(hipGetLastError());
     DevBuf<int64_t> chunk_off, bucket_off;
     RG_TRY(chunk_off.alloc(chunk_tot + 1, s));
-    RG_TRY(bucket_off.alloc(4 * cell_tot + 1, s));
+    RG_TRY(bucket_off.alloc(4 * n_groups + 1, s));
     RG_TRY(scan_i32_to_i64(count.p, chunk_off.p, chunk_tot, s));
-    RG_TRY(scan_i32_to_i64(hist.p, bucket_off.p, 4 * cell_tot, s));
+    RG_TRY(scan_i32_to_i64(hist.p, bucket_off.p, 4 * n_groups, s));
     int64_t tot[2];
     int32_t cnt[4];
     RG_TRY(read_scalars(chunk_off.p + chunk_tot, &tot[0], sizeof(int64_t), s));
-    RG_TRY(read_scalars(bucket_off.p + 4 * cell_tot, &tot[1], sizeof(int64_t), s));
+    RG_TRY(read_scalars(bucket_off.p + 4 * n_groups, &tot[1], sizeof(int64_t), s));
     RG_TRY(read_scalars(counters.p, cnt, sizeof(cnt), s));
     up.pending = false;  // the stream has passed the upload
     tm.mark();  // 4
@@ -1390,45 +1466,47 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
 
     // ---- emit
     DevBuf<KV> rec;
+    DevBuf<uint8_t> rcell;
     RG_TRY(rec.alloc(T, s));
+    RG_TRY(rcell.alloc(T, s));
     uint32_t* cursor = (uint32_t*)hist.p;  // reused: one word per bucket
-    k_cursor_init<<<grid_for(4 * cell_tot, 256), 256, 0, s>>>(bucket_off.p, cursor,
-                                                              4 * cell_tot);
+    k_cursor_init<<<grid_for(4 * n_groups, 256), 256, 0, s>>>(bucket_off.p, cursor,
+                                                              4 * n_groups);
     tm.mark();  // 5
     k_scatter<<<grid_for(thr_tot, block), block, 0, s>>>(
-        d_builds, pair_of_wave.p, thr_tot, cell_tot, start.p, count.p, stage.p, cap,
-        chunk_off.p, cursor, rec.p, (uint32_t)T, counters.p + 2, counters.p + 4);
+        d_builds, pair_of_wave.p, thr_tot, n_groups, start.p, count.p, stage.p, cap,
+        chunk_off.p, cursor, rec.p, rcell.p, (uint32_t)T, counters.p + 2, counters.p + 4);
     RG_HIP(hipGetLastError());
     tm.mark();  // 6
-    k_cursor_check<<<grid_for(4 * cell_tot, 256), 256, 0, s>>>(
-        bucket_off.p, cursor, 4 * cell_tot, counters.p + 2);
+    k_cursor_check<<<grid_for(4 * n_groups, 256), 256, 0, s>>>(
+        bucket_off.p, cursor, 4 * n_groups, counters.p + 2);
     RG_HIP(hipGetLastError());
     stage.reset();
 
     // ---- reduce -> CSR by input
     DevBuf<int32_t> nnz_row, long_list, group_mode;
-    DevBuf<int64_t> row_off, cell_off;
+    DevBuf<int64_t> row_off, cell_off, group_off;
     DevBuf<KV> red;
-    const int64_t n_groups = cell_tot / 64;
     RG_TRY(nnz_row.alloc(cell_tot, s));
     RG_TRY(long_list.alloc(cell_tot, s));
     RG_TRY(group_mode.alloc(n_groups, s));
     RG_TRY(row_off.alloc(cell_tot + 1, s));
     RG_TRY(cell_off.alloc(cell_tot + 1, s));
+    RG_TRY(group_off.alloc(n_groups + 1, s));
     RG_TRY(red.alloc(T, s));
-    // cell-major offsets of the reduction's output (nnz_row as scratch)
-    k_cell_counts<<<grid_for(cell_tot, 256), 256, 0, s>>>(bucket_off.p, cell_tot,
-                                                          nnz_row.p);
+    // group-major offsets of the reduction's output (group_mode as scratch)
+    k_group_counts<<<grid_for(n_groups, 256), 256, 0, s>>>(bucket_off.p, n_groups,
+                                                           group_mode.p);
     RG_HIP(hipGetLastError());
-    RG_TRY(scan_i32_to_i64(nnz_row.p, cell_off.p, cell_tot, s));
+    RG_TRY(scan_i32_to_i64(group_mode.p, group_off.p, n_groups, s));
     tm.mark();  // 7
     k_reduce_groups<<<(unsigned)((n_groups + RED_WAVES - 1) / RED_WAVES),
-                      RED_WAVES * 64, 0, s>>>(bucket_off.p, cell_tot, cell_off.p, rec.p,
-                                              red.p, vol.p, wi_all.p, nnz_row.p,
-                                              group_mode.p, long_list.p, counters.p + 3);
-    k_bucket_long<<<256, 64, 0, s>>>(bucket_off.p, cell_tot, cell_off.p, rec.p, red.p,
-                                   vol.p, wi_all.p, long_list.p, counters.p + 3,
-                                   nnz_row.p);
+                      RED_WAVES * 64, 0, s>>>(bucket_off.p, n_groups, group_off.p, rec.p,
+                                              rcell.p, red.p, cell_off.p, vol.p, wi_all.p,
+                                              nnz_row.p, group_mode.p, long_list.p,
+                                              counters.p + 3);
+    k_bucket_long<<<256, 64, 0, s>>>(cell_off.p, red.p, vol.p, wi_all.p, long_list.p,
+                                   counters.p + 3, nnz_row.p);
     RG_HIP(hipGetLastError());
     tm.mark();  // 8
     RG_TRY(scan_i32_to_i64(nnz_row.p, row_off.p, cell_tot, s));
