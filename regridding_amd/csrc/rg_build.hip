@@ -129,25 +129,23 @@ __device__ inline void put_record(KV* __restrict__ rec, uint8_t* __restrict__ rc
     rcell[pos] = (uint8_t)(cin & 63);
 }
 
-// Writes each triple into its bucket with its own claim (chunks whose
-// triples overflowed the staging buffer walk again with this sink).
-struct BucketSink {
-    uint32_t ordinal;
-    uint32_t* __restrict__ cursor;
-    KV* __restrict__ rec;
-    uint8_t* __restrict__ rcell;
-    uint32_t n_rec;
-    int32_t* __restrict__ overflow;
-    int32_t p;
-    int64_t ps;
+// Keeps the triples beyond the first `cap` of a chunk (the ones the
+// counting walk could not stage) in the chunk's range of the overflow
+// buffer: its first triple is the chunk's triple number `cap`.
+struct OverflowSink {
+    int32_t n;
+    int64_t cap;
+    Triple* __restrict__ out;
     __device__ void emit(int64_t cin, int64_t cout, double a)
     {
-        const uint32_t pos = atomicAdd(&cursor[p * ps + (cin >> 6)], 1u);
-        if (pos < n_rec)
-            put_record(rec, rcell, pos, cin, cout, ordinal, a);
-        else
-            atomicAdd(overflow, 1);
-        ordinal++;
+        if (n >= cap) {
+            Triple t;
+            t.cin = (int32_t)cin;
+            t.cout = (int32_t)cout;
+            t.a = a;
+            out[n - cap] = t;
+        }
+        n++;
     }
 };
 
@@ -562,6 +560,49 @@ __global__ __launch_bounds__(WALK_MAX_BLOCK) void k_fixup(const Build* __restric
     if (fixes) atomicAdd(nfix, fixes);
 }
 
+// Triples beyond the staging capacity, per chunk (then scanned into the
+// chunks' ranges of the overflow buffer).
+__global__ void k_overflow_counts(const int32_t* __restrict__ count, int64_t nchunks,
+                                  int64_t cap, int32_t* __restrict__ ov,
+                                  unsigned long long* __restrict__ total)
+{
+    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t v = 0;
+    if (g < nchunks) {
+        v = count[g] > cap ? count[g] - cap : 0;
+        ov[g] = (int32_t)v;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(total, (unsigned long long)v);
+}
+
+// Walks every chunk that overflowed the staging again and keeps the
+// triples the staging could not hold.
+__global__ __launch_bounds__(WALK_MAX_BLOCK) void k_restage(const Build* __restrict__ builds,
+                          const int32_t* __restrict__ pair_of_wave,
+                          int64_t nthreads_all,
+                          const int32_t* __restrict__ start,
+                          const int32_t* __restrict__ count,
+                          const int64_t* __restrict__ ov_off,
+                          Triple* __restrict__ overflow, int64_t cap,
+                          int32_t* __restrict__ restaged)
+{
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= nthreads_all) return;
+    const Build& B = build_of_thread(builds, pair_of_wave, t);
+    const ChunkRef r = chunk_of_thread(B, t - B.thread_gbase);
+    if (!r.valid) return;
+    const int64_t g = B.chunk_gbase + r.g;
+    if (count[g] <= cap) return;
+    const Pass& P = B.pass[r.p];
+    int64_t k0, k1;
+    chunk_range(B, P, r.c, &k0, &k1);
+    OverflowSink sink{0, cap, overflow + ov_off[g]};
+    (void)walk_chunk(B, P, r.row, k0, k1, start[g], sink);
+    atomicAdd(restaged, 1);
+}
+
 // Orders the LDS accesses of a wavefront's lanes before and after it.
 __device__ inline void wave_sync_lds()
 {
@@ -577,8 +618,8 @@ __device__ inline void wave_sync_lds()
 // round: bucket, count, claimed base); one lane per entry claims the
 // bucket's records with a single atomic, and the wave writes every
 // bucket's records as one contiguous run.  A full table falls back to one
-// claim per triple.  Chunks whose triples overflowed the staging walk again,
-// emitting straight into the buckets.
+// claim per triple.  A chunk's triples beyond the staging capacity come
+// from the overflow buffer (k_restage).
 constexpr int SCAT_R = 8;     // triples per lane per round
 constexpr int SCAT_HT = 256;  // hash entries per wavefront
 
@@ -591,15 +632,15 @@ struct ScatTable {
 __global__ __launch_bounds__(WALK_MAX_BLOCK) void k_scatter(const Build* __restrict__ builds,
                           const int32_t* __restrict__ pair_of_wave,
                           int64_t nthreads_all, int64_t ngroups_all,
-                          const int32_t* __restrict__ start,
                           const int32_t* __restrict__ count,
                           const Triple* __restrict__ stage, int64_t cap,
+                          const Triple* __restrict__ ovbuf,
+                          const int64_t* __restrict__ ov_off,
                           const int64_t* __restrict__ chunk_off,
                           uint32_t* __restrict__ cursor_all,
                           KV* __restrict__ rec, uint8_t* __restrict__ rcell,
                           uint32_t n_rec,
-                          int32_t* __restrict__ overflow,
-                          int32_t* __restrict__ restaged)
+                          int32_t* __restrict__ overflow)
 {
     __shared__ ScatTable tables[WALK_MAX_BLOCK / 64];
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -613,8 +654,8 @@ __global__ __launch_bounds__(WALK_MAX_BLOCK) void k_scatter(const Build* __restr
     uint32_t* cursor = cursor_all + (B.cell_gbase >> 6) + (int64_t)r.p * ngroups_all;
     const int32_t n = r.valid ? count[g] : 0;
     const uint32_t ord = r.valid ? (uint32_t)chunk_off[g] : 0;
-    const int32_t nst = n > cap ? 0 : n;  // staged triples this lane scatters
-    int32_t mx = nst;
+    const Triple* ov = n > cap ? ovbuf + ov_off[g] : nullptr;
+    int32_t mx = n;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
     for (int h = lane; h < SCAT_HT; h += 64) {
@@ -628,8 +669,8 @@ __global__ __launch_bounds__(WALK_MAX_BLOCK) void k_scatter(const Build* __restr
 #pragma unroll
         for (int k = 0; k < SCAT_R; k++) {
             slot[k] = -2;  // no triple
-            if (q0 + k < nst) {
-                e[k] = stage[stage_index(t, q0 + k, cap)];
+            if (q0 + k < n) {
+                e[k] = q0 + k < cap ? stage[stage_index(t, q0 + k, cap)] : ov[q0 + k - cap];
                 const int32_t b = e[k].cin >> 6;
                 int32_t h = b & (SCAT_HT - 1);
                 slot[k] = -1;  // table full: claim alone
@@ -666,15 +707,6 @@ __global__ __launch_bounds__(WALK_MAX_BLOCK) void k_scatter(const Build* __restr
             H.cnt[h] = 0;
         }
         wave_sync_lds();
-    }
-    if (r.valid && n > cap) {
-        const Pass& P = B.pass[r.p];
-        int64_t k0, k1;
-        chunk_range(B, P, r.c, &k0, &k1);
-        BucketSink sink{ord, cursor_all + (B.cell_gbase >> 6), rec, rcell, n_rec, overflow,
-                        r.p, ngroups_all};
-        (void)walk_chunk(B, P, r.row, k0, k1, start[g], sink);
-        atomicAdd(restaged, 1);
     }
 }
 
@@ -1413,7 +1445,8 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
     RG_TRY(end.alloc(chunk_tot, s));
     RG_TRY(count.alloc(chunk_tot, s));
     RG_TRY(hist.alloc(4 * n_groups, s));
-    // anomalies, fixups, failed/overflow, long runs, restaged chunks
+    // anomalies, fixups, failed/overflow, long runs, restaged chunks; words
+    // 6-7: the triples beyond the staging capacity (64-bit)
     RG_TRY(counters.alloc(8, s));
     RG_TRY(fill_zero(hist.p, 4 * n_groups, s));
     RG_TRY(fill_zero(counters.p, 8, s));
@@ -1445,11 +1478,19 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
     RG_TRY(bucket_off.alloc(4 * n_groups + 1, s));
     RG_TRY(scan_i32_to_i64(count.p, chunk_off.p, chunk_tot, s));
     RG_TRY(scan_i32_to_i64(hist.p, bucket_off.p, 4 * n_groups, s));
+    // triples beyond the staging capacity, per chunk and in all
+    DevBuf<int32_t> ov;
+    RG_TRY(ov.alloc(chunk_tot, s));
+    k_overflow_counts<<<grid_for(chunk_tot, 256), 256, 0, s>>>(
+        count.p, chunk_tot, cap, ov.p, (unsigned long long*)(counters.p + 6));
+    RG_HIP(hipGetLastError());
     int64_t tot[2];
-    int32_t cnt[4];
+    int32_t cnt[8];
     RG_TRY(read_scalars(chunk_off.p + chunk_tot, &tot[0], sizeof(int64_t), s));
     RG_TRY(read_scalars(bucket_off.p + 4 * n_groups, &tot[1], sizeof(int64_t), s));
     RG_TRY(read_scalars(counters.p, cnt, sizeof(cnt), s));
+    int64_t ov_tot;
+    std::memcpy(&ov_tot, cnt + 6, sizeof(ov_tot));
     up.pending = false;  // the stream has passed the upload
     tm.mark();  // 4
     if (cnt[2] != 0)
@@ -1473,9 +1514,21 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
     k_cursor_init<<<grid_for(4 * n_groups, 256), 256, 0, s>>>(bucket_off.p, cursor,
                                                               4 * n_groups);
     tm.mark();  // 5
+    DevBuf<Triple> ovbuf;
+    DevBuf<int64_t> ov_off;
+    RG_TRY(ovbuf.alloc(ov_tot > 0 ? ov_tot : 1, s));
+    RG_TRY(ov_off.alloc(chunk_tot + 1, s));
+    if (ov_tot > 0) {
+        // the chunks' ranges of the overflow buffer
+        RG_TRY(scan_i32_to_i64(ov.p, ov_off.p, chunk_tot, s));
+        k_restage<<<grid_for(thr_tot, block), block, 0, s>>>(
+            d_builds, pair_of_wave.p, thr_tot, start.p, count.p, ov_off.p, ovbuf.p, cap,
+            counters.p + 4);
+        RG_HIP(hipGetLastError());
+    }
     k_scatter<<<grid_for(thr_tot, block), block, 0, s>>>(
-        d_builds, pair_of_wave.p, thr_tot, n_groups, start.p, count.p, stage.p, cap,
-        chunk_off.p, cursor, rec.p, rcell.p, (uint32_t)T, counters.p + 2, counters.p + 4);
+        d_builds, pair_of_wave.p, thr_tot, n_groups, count.p, stage.p, cap, ovbuf.p,
+        ov_off.p, chunk_off.p, cursor, rec.p, rcell.p, (uint32_t)T, counters.p + 2);
     RG_HIP(hipGetLastError());
     tm.mark();  // 6
     k_cursor_check<<<grid_for(4 * n_groups, 256), 256, 0, s>>>(
