@@ -726,6 +726,33 @@ __global__ __launch_bounds__(WALK_MAX_BLOCK) void k_scatter(const Build* __restr
 // k_bucket_long.
 constexpr int RED_WAVES = 2;     // wavefronts per block
 constexpr int RED_CAP = 1024;    // LDS entries per wavefront (16 KiB)
+constexpr int RED_NET = 16;      // largest cell sorted in registers
+
+// One cell's terms: divided by the cell area (and weighted), sorted by key
+// in registers through a network of N, written back in place.
+template <int N>
+__device__ inline void sort_cell_in_registers(uint64_t* K, double* V, int32_t n, double vol,
+                                              const double* wi, int64_t i)
+{
+    uint64_t k[N];
+    double v[N];
+#pragma unroll
+    for (int j = 0; j < N; j++) {
+        k[j] = ~(uint64_t)0;  // padding sorts last
+        v[j] = 0;
+        if (j < n) {
+            k[j] = K[j];
+            v[j] = weight_of(V[j], vol, wi, i);
+        }
+    }
+    sort_network<N>(k, v);
+#pragma unroll
+    for (int j = 0; j < N; j++)
+        if (j < n) {
+            K[j] = k[j];
+            V[j] = v[j];
+        }
+}
 
 // Records of one group (all passes): the group totals that give the
 // group-major output offsets.
@@ -831,8 +858,20 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
             }
         wave_sync_lds();
         bool lr = false;
-        for (int32_t q = 0; q < n; q++) V[s + q] = weight_of(V[s + q], vol, wi, i);
-        int64_t m = bucket_reduce(K + s, V + s, n, &lr);
+        // cells of up to RED_NET records sort in registers (a sorting
+        // network: no LDS round trips); larger ones sort in LDS
+        int32_t nmax = n;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o, 64));
+        if (nmax <= RED_NET / 2) {
+            sort_cell_in_registers<RED_NET / 2>(K + s, V + s, n, vol, wi, i);
+        } else if (n <= RED_NET) {
+            sort_cell_in_registers<RED_NET>(K + s, V + s, n, vol, wi, i);
+        } else {
+            for (int32_t q = 0; q < n; q++) V[s + q] = weight_of(V[s + q], vol, wi, i);
+            sort_pairs(K + s, V + s, (int64_t)n);
+        }
+        int64_t m = bucket_reduce_sorted(K + s, V + s, n, &lr);
         if (lr) m = n;  // left sorted, uncompacted, for k_bucket_long
         const bool packed = __ballot(lr) == 0;
         int64_t dst = s;  // a group with a long run keeps cell starts

@@ -701,10 +701,9 @@ RG_HD void sort_records(KV* r, int64_t n)
 // overlap in a polygon of a few edges; if one does, the bucket is left
 // sorted but not compacted, *long_run is set, and bucket_reduce_long (run by
 // a separate kernel) finishes it.
-RG_HD int64_t bucket_reduce(uint64_t* keys, double* vals, int64_t n,
-                            bool* long_run)
+RG_HD int64_t bucket_reduce_sorted(uint64_t* keys, double* vals, int64_t n,
+                                   bool* long_run)
 {
-    sort_pairs(keys, vals, n);
     int64_t m = 0, p = 0;
     bool lr = false;
     while (p < n) {
@@ -730,6 +729,41 @@ RG_HD int64_t bucket_reduce(uint64_t* keys, double* vals, int64_t n,
         p = q;
     }
     return m;
+}
+
+RG_HD int64_t bucket_reduce(uint64_t* keys, double* vals, int64_t n,
+                            bool* long_run)
+{
+    sort_pairs(keys, vals, n);
+    return bucket_reduce_sorted(keys, vals, n, long_run);
+}
+
+// Bitonic sorting network over N keys in registers (N a power of two, every
+// index static once unrolled), values following.
+template <int N>
+RG_HD inline void sort_network(uint64_t* k, double* v)
+{
+#pragma unroll
+    for (int size = 2; size <= N; size <<= 1) {
+#pragma unroll
+        for (int stride = size / 2; stride > 0; stride >>= 1) {
+#pragma unroll
+            for (int i = 0; i < N; i++) {
+                const int j = i ^ stride;
+                if (j > i) {
+                    const bool up = (i & size) == 0;
+                    if ((k[i] > k[j]) == up) {
+                        const uint64_t tk = k[i];
+                        k[i] = k[j];
+                        k[j] = tk;
+                        const double tv = v[i];
+                        v[i] = v[j];
+                        v[j] = tv;
+                    }
+                }
+            }
+        }
+    }
 }
 
 // Completion of a bucket left sorted (KV records, weights already divided)
