@@ -768,6 +768,41 @@ __global__ void k_group_counts(const int64_t* __restrict__ bucket_off, int64_t n
     cnt[g] = (int32_t)t;
 }
 
+// The records of one group, reached through its four pass ranges.
+struct GroupRecs {
+    int64_t ps0, ps1, ps2, ps3, e0, e1, e2;
+    __device__ GroupRecs(const int64_t* __restrict__ bucket_off, int64_t n_groups, int64_t grp)
+    {
+        ps0 = bucket_off[grp];
+        ps1 = bucket_off[n_groups + grp];
+        ps2 = bucket_off[2 * n_groups + grp];
+        ps3 = bucket_off[3 * n_groups + grp];
+        e0 = bucket_off[grp + 1] - ps0;
+        e1 = e0 + bucket_off[n_groups + grp + 1] - ps1;
+        e2 = e1 + bucket_off[2 * n_groups + grp + 1] - ps2;
+    }
+    // position of record q of the concatenated pass ranges
+    __device__ int64_t src(int64_t q) const
+    {
+        return q < e0 ? ps0 + q : q < e1 ? ps1 + (q - e0) : q < e2 ? ps2 + (q - e1)
+                                                               : ps3 + (q - e2);
+    }
+};
+
+// Exclusive prefix over the wave of one value per lane.
+__device__ inline int32_t wave_exclusive(int32_t v, int lane)
+{
+    int32_t incl = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int32_t y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    return incl - v;
+}
+
+// Groups whose records fit the LDS share; a larger group is listed for
+// k_reduce_heavy.
 __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
     const int64_t* __restrict__ bucket_off, int64_t n_groups,
     const int64_t* __restrict__ group_off, const KV* __restrict__ rec,
@@ -775,131 +810,149 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
     int64_t* __restrict__ cell_off, const double* __restrict__ vol_in,
     const double* __restrict__ wi, int32_t* __restrict__ nnz_row,
     int32_t* __restrict__ group_mode, int32_t* __restrict__ long_list,
-    int32_t* __restrict__ n_long)
+    int32_t* __restrict__ n_long, int32_t* __restrict__ heavy_list,
+    int32_t* __restrict__ n_heavy)
 {
     __shared__ uint64_t sk[RED_WAVES][RED_CAP];
     __shared__ double sv[RED_WAVES][RED_CAP];
     __shared__ int32_t scnt[RED_WAVES][64];
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (blockIdx.x == 0 && threadIdx.x == 0) cell_off[n_groups * 64] = group_off[n_groups];
     const int64_t grp = (int64_t)blockIdx.x * RED_WAVES + wid;
     if (grp >= n_groups) return;  // uniform per wavefront; no block barriers below
-    const int64_t i = grp * 64 + lane;
     const int64_t base = group_off[grp];
     const int64_t total = group_off[grp + 1] - base;
+    if (total > RED_CAP) {
+        if (lane == 0) heavy_list[atomicAdd(n_heavy, 1)] = (int32_t)grp;
+        return;
+    }
+    const int64_t i = grp * 64 + lane;
     int32_t* C = scnt[wid];
-    // the group's pass ranges [ps_p, ps_p + len_p), concatenated: record q
-    // of the group sits at src_of(q)
-    const int64_t ps0 = bucket_off[grp], ps1 = bucket_off[n_groups + grp];
-    const int64_t ps2 = bucket_off[2 * n_groups + grp], ps3 = bucket_off[3 * n_groups + grp];
-    const int64_t e0 = bucket_off[grp + 1] - ps0;
-    const int64_t e1 = e0 + bucket_off[n_groups + grp + 1] - ps1;
-    const int64_t e2 = e1 + bucket_off[2 * n_groups + grp + 1] - ps2;
-    auto src_of = [=](int64_t q) -> int64_t {
-        return q < e0 ? ps0 + q : q < e1 ? ps1 + (q - e0) : q < e2 ? ps2 + (q - e1)
-                                                               : ps3 + (q - e2);
-    };
-    const bool fits = total <= RED_CAP;
-    // a group that fits: its records are read once, into registers (record
-    // q = lane + 64 u of the concatenated pass ranges)
+    const GroupRecs G(bucket_off, n_groups, grp);
+    // the records are read once, into registers (record q = lane + 64 u of
+    // the concatenated pass ranges)
     KV rr[RED_CAP / 64];
     int32_t cc[RED_CAP / 64];
     C[lane] = 0;
     wave_sync_lds();
-    if (fits) {
 #pragma unroll
-        for (int u = 0; u < RED_CAP / 64; u++) {
-            const int64_t q = lane + 64 * u;
-            cc[u] = -1;
-            if (q < total) {
-                const int64_t src = src_of(q);
-                cc[u] = rcell[src];
-                rr[u] = rec[src];
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < RED_CAP / 64; u++)
-            if (cc[u] >= 0) atomicAdd(&C[cc[u]], 1);
-    } else {
-        for (int64_t q0 = lane; q0 < total; q0 += 256) {
-            int32_t c4[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++)
-                c4[u] = q0 + 64 * u < total ? rcell[src_of(q0 + 64 * u)] : -1;
-#pragma unroll
-            for (int u = 0; u < 4; u++)
-                if (c4[u] >= 0) atomicAdd(&C[c4[u]], 1);
+    for (int u = 0; u < RED_CAP / 64; u++) {
+        const int64_t q = lane + 64 * u;
+        cc[u] = -1;
+        if (q < total) {
+            const int64_t src = G.src(q);
+            cc[u] = rcell[src];
+            rr[u] = rec[src];
         }
     }
+#pragma unroll
+    for (int u = 0; u < RED_CAP / 64; u++)
+        if (cc[u] >= 0) atomicAdd(&C[cc[u]], 1);
     wave_sync_lds();
     // records per cell, and the cell-major offsets
     const int32_t n = C[lane];
-    int32_t incl = n;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int32_t y = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += y;
-    }
-    const int32_t s = incl - n;
+    const int32_t s = wave_exclusive(n, lane);
     cell_off[i] = base + s;
-    if (grp == n_groups - 1 && lane == 63) cell_off[i + 1] = base + total;
     const double vol = vol_in[i];
     wave_sync_lds();
     C[lane] = s;  // placement cursors
     wave_sync_lds();
     uint64_t* K = sk[wid];
     double* V = sv[wid];
-    if (fits) {
 #pragma unroll
-        for (int u = 0; u < RED_CAP / 64; u++)
-            if (cc[u] >= 0) {
-                const int32_t d = atomicAdd(&C[cc[u]], 1);
-                K[d] = rr[u].key;
-                V[d] = rr[u].val;
-            }
-        wave_sync_lds();
-        bool lr = false;
-        // cells of up to RED_NET records sort in registers (a sorting
-        // network: no LDS round trips); larger ones sort in LDS
-        int32_t nmax = n;
+    for (int u = 0; u < RED_CAP / 64; u++)
+        if (cc[u] >= 0) {
+            const int32_t d = atomicAdd(&C[cc[u]], 1);
+            K[d] = rr[u].key;
+            V[d] = rr[u].val;
+        }
+    wave_sync_lds();
+    // cells of up to RED_NET records sort in registers (a sorting network:
+    // no LDS round trips); larger ones sort in LDS
+    int32_t nmax = n;
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o, 64));
-        if (nmax <= RED_NET / 2) {
-            sort_cell_in_registers<RED_NET / 2>(K + s, V + s, n, vol, wi, i);
-        } else if (n <= RED_NET) {
-            sort_cell_in_registers<RED_NET>(K + s, V + s, n, vol, wi, i);
-        } else {
-            for (int32_t q = 0; q < n; q++) V[s + q] = weight_of(V[s + q], vol, wi, i);
-            sort_pairs(K + s, V + s, (int64_t)n);
-        }
-        int64_t m = bucket_reduce_sorted(K + s, V + s, n, &lr);
-        if (lr) m = n;  // left sorted, uncompacted, for k_bucket_long
-        const bool packed = __ballot(lr) == 0;
-        int64_t dst = s;  // a group with a long run keeps cell starts
-        if (packed) {
-            // exclusive prefix of m over the wave: packed destination
-            int64_t mi = m;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const int64_t y = __shfl_up(mi, o, 64);
-                if (lane >= o) mi += y;
-            }
-            dst = mi - m;
-        }
-        for (int64_t t = 0; t < m; t++) {
-            KV r;
-            r.key = K[s + t];
-            r.val = V[s + t];
-            red[base + dst + t] = r;
-        }
-        nnz_row[i] = (int32_t)m;
-        if (lr) long_list[atomicAdd(n_long, 1)] = (int32_t)i;
-        if (lane == 0) group_mode[grp] = packed ? 0 : 1;
+    for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o, 64));
+    if (nmax <= RED_NET / 2) {
+        sort_cell_in_registers<RED_NET / 2>(K + s, V + s, n, vol, wi, i);
+    } else if (n <= RED_NET) {
+        sort_cell_in_registers<RED_NET>(K + s, V + s, n, vol, wi, i);
     } else {
+        for (int32_t q = 0; q < n; q++) V[s + q] = weight_of(V[s + q], vol, wi, i);
+        sort_pairs(K + s, V + s, (int64_t)n);
+    }
+    bool lr = false;
+    int64_t m = bucket_reduce_sorted(K + s, V + s, n, &lr);
+    if (lr) m = n;  // left sorted, uncompacted, for k_bucket_long
+    const bool packed = __ballot(lr) == 0;
+    // packed destination: exclusive prefix of m over the wave; a group
+    // with a long run keeps the cell starts
+    const int64_t dst = packed ? wave_exclusive((int32_t)m, lane) : s;
+    for (int64_t t = 0; t < m; t++) {
+        KV r;
+        r.key = K[s + t];
+        r.val = V[s + t];
+        red[base + dst + t] = r;
+    }
+    nnz_row[i] = (int32_t)m;
+    if (lr) long_list[atomicAdd(n_long, 1)] = (int32_t)i;
+    if (lane == 0) group_mode[grp] = packed ? 0 : 1;
+}
+
+// Groups beyond the LDS share, one wavefront each (grid-stride over the
+// list): the group is copied cell-major into `red` (global), then goes
+// through LDS in runs of consecutive cells, each cell reduced in place at
+// its own offset (the group is marked unpacked); a single cell beyond the
+// share is left to k_bucket_long.  Cells of up to 2 RED_NET records sort in
+// registers.
+__global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_heavy(
+    const int64_t* __restrict__ bucket_off, int64_t n_groups,
+    const int64_t* __restrict__ group_off, const KV* __restrict__ rec,
+    const uint8_t* __restrict__ rcell, KV* __restrict__ red,
+    int64_t* __restrict__ cell_off, const double* __restrict__ vol_in,
+    const double* __restrict__ wi, int32_t* __restrict__ nnz_row,
+    int32_t* __restrict__ group_mode, int32_t* __restrict__ long_list,
+    int32_t* __restrict__ n_long, const int32_t* __restrict__ heavy_list,
+    const int32_t* __restrict__ n_heavy)
+{
+    __shared__ uint64_t sk[RED_WAVES][RED_CAP];
+    __shared__ double sv[RED_WAVES][RED_CAP];
+    __shared__ int32_t scnt[RED_WAVES][64];
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int32_t* C = scnt[wid];
+    uint64_t* K = sk[wid];
+    double* V = sv[wid];
+    const int32_t nh = *n_heavy;
+    for (int64_t h = (int64_t)blockIdx.x * RED_WAVES + wid; h < nh;
+         h += (int64_t)gridDim.x * RED_WAVES) {  // uniform per wavefront
+        const int64_t grp = heavy_list[h];
+        const int64_t i = grp * 64 + lane;
+        const int64_t base = group_off[grp];
+        const int64_t total = group_off[grp + 1] - base;
+        const GroupRecs G(bucket_off, n_groups, grp);
+        C[lane] = 0;
+        wave_sync_lds();
+        for (int64_t q0 = lane; q0 < total; q0 += 256) {
+            int32_t c4[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                c4[u] = q0 + 64 * u < total ? rcell[G.src(q0 + 64 * u)] : -1;
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (c4[u] >= 0) atomicAdd(&C[c4[u]], 1);
+        }
+        wave_sync_lds();
+        const int32_t n = C[lane];
+        const int32_t s = wave_exclusive(n, lane);
+        cell_off[i] = base + s;
+        const double vol = vol_in[i];
+        wave_sync_lds();
+        C[lane] = s;  // placement cursors
+        wave_sync_lds();
         // cell-major copy of the group in `red`, read back by the wave's lanes
         for (int64_t q0 = lane; q0 < total; q0 += 128) {
             // two records in flight per lane
             const int64_t qa = q0, qb = q0 + 64;
-            const int64_t sa = src_of(qa), sb2 = qb < total ? src_of(qb) : sa;
+            const int64_t sa = G.src(qa), sb2 = qb < total ? G.src(qb) : sa;
             const int32_t ca = rcell[sa];
             const KV ra = rec[sa];
             const int32_t cb = rcell[sb2];
@@ -933,9 +986,16 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
             if (in) {
                 uint64_t* Kc = K + (s - sb);
                 double* Vc = V + (s - sb);
-                for (int32_t q = 0; q < n; q++) Vc[q] = weight_of(Vc[q], vol, wi, i);
+                if (n <= RED_NET) {
+                    sort_cell_in_registers<RED_NET>(Kc, Vc, n, vol, wi, i);
+                } else if (n <= 2 * RED_NET) {
+                    sort_cell_in_registers<2 * RED_NET>(Kc, Vc, n, vol, wi, i);
+                } else {
+                    for (int32_t q = 0; q < n; q++) Vc[q] = weight_of(Vc[q], vol, wi, i);
+                    sort_pairs(Kc, Vc, (int64_t)n);
+                }
                 bool lrc = false;
-                int64_t m = bucket_reduce(Kc, Vc, n, &lrc);
+                int64_t m = bucket_reduce_sorted(Kc, Vc, n, &lrc);
                 if (lrc) m = n;  // left sorted, uncompacted, for k_bucket_long
                 for (int64_t t = 0; t < m; t++) {
                     KV r;
@@ -950,6 +1010,7 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
             c = d;
         }
         if (lane == 0) group_mode[grp] = 1;
+        wave_sync_lds();  // LDS reuse by the next group
     }
 }
 
@@ -1592,11 +1653,17 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
     RG_HIP(hipGetLastError());
     RG_TRY(scan_i32_to_i64(group_mode.p, group_off.p, n_groups, s));
     tm.mark();  // 7
+    DevBuf<int32_t> heavy_list;
+    RG_TRY(heavy_list.alloc(n_groups, s));
     k_reduce_groups<<<(unsigned)((n_groups + RED_WAVES - 1) / RED_WAVES),
                       RED_WAVES * 64, 0, s>>>(bucket_off.p, n_groups, group_off.p, rec.p,
                                               rcell.p, red.p, cell_off.p, vol.p, wi_all.p,
                                               nnz_row.p, group_mode.p, long_list.p,
-                                              counters.p + 3);
+                                              counters.p + 3, heavy_list.p, counters.p + 5);
+    k_reduce_heavy<<<1024, RED_WAVES * 64, 0, s>>>(
+        bucket_off.p, n_groups, group_off.p, rec.p, rcell.p, red.p, cell_off.p, vol.p,
+        wi_all.p, nnz_row.p, group_mode.p, long_list.p, counters.p + 3, heavy_list.p,
+        counters.p + 5);
     k_bucket_long<<<256, 64, 0, s>>>(cell_off.p, red.p, vol.p, wi_all.p, long_list.p,
                                    counters.p + 3, nnz_row.p);
     RG_HIP(hipGetLastError());
