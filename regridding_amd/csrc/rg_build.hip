@@ -768,6 +768,19 @@ __global__ void k_group_counts(const int64_t* __restrict__ bucket_off, int64_t n
     cnt[g] = (int32_t)t;
 }
 
+// One reduced entry (output cell `out`, weight) of a group: stored with its
+// arrival rank among the entries of its output row (the high word of the
+// key), counted in hist_out for the transposed CSR.
+__device__ inline void put_reduced(KV* __restrict__ dst, uint64_t out, double w,
+                                   int32_t* __restrict__ hist_out)
+{
+    const uint32_t rank = (uint32_t)atomicAdd(&hist_out[(int64_t)out], 1);
+    KV r;
+    r.key = out | ((uint64_t)rank << 32);
+    r.val = w;
+    *dst = r;
+}
+
 // The records of one group, reached through its four pass ranges.
 struct GroupRecs {
     int64_t ps0, ps1, ps2, ps3, e0, e1, e2;
@@ -811,7 +824,8 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
     const double* __restrict__ wi, int32_t* __restrict__ nnz_row,
     int32_t* __restrict__ group_mode, int32_t* __restrict__ long_list,
     int32_t* __restrict__ n_long, int32_t* __restrict__ heavy_list,
-    int32_t* __restrict__ n_heavy)
+    int32_t* __restrict__ n_heavy, const PairCells* __restrict__ pcs,
+    const int32_t* __restrict__ pair_of_group, int32_t* __restrict__ hist_out_all)
 {
     __shared__ uint64_t sk[RED_WAVES][RED_CAP];
     __shared__ double sv[RED_WAVES][RED_CAP];
@@ -887,11 +901,17 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
     // packed destination: exclusive prefix of m over the wave; a group
     // with a long run keeps the cell starts
     const int64_t dst = packed ? wave_exclusive((int32_t)m, lane) : s;
-    for (int64_t t = 0; t < m; t++) {
-        KV r;
-        r.key = K[s + t];
-        r.val = V[s + t];
-        red[base + dst + t] = r;
+    if (lr) {
+        for (int64_t t = 0; t < m; t++) {  // finished by k_bucket_long
+            KV r;
+            r.key = K[s + t];
+            r.val = V[s + t];
+            red[base + dst + t] = r;
+        }
+    } else {
+        int32_t* hist_out =
+            hist_out_all + pcs[__builtin_amdgcn_readfirstlane(pair_of_group[grp])].out_gbase;
+        for (int64_t t = 0; t < m; t++) put_reduced(red + base + dst + t, K[s + t], V[s + t], hist_out);
     }
     nnz_row[i] = (int32_t)m;
     if (lr) long_list[atomicAdd(n_long, 1)] = (int32_t)i;
@@ -912,7 +932,8 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_heavy(
     const double* __restrict__ wi, int32_t* __restrict__ nnz_row,
     int32_t* __restrict__ group_mode, int32_t* __restrict__ long_list,
     int32_t* __restrict__ n_long, const int32_t* __restrict__ heavy_list,
-    const int32_t* __restrict__ n_heavy)
+    const int32_t* __restrict__ n_heavy, const PairCells* __restrict__ pcs,
+    const int32_t* __restrict__ pair_of_group, int32_t* __restrict__ hist_out_all)
 {
     __shared__ uint64_t sk[RED_WAVES][RED_CAP];
     __shared__ double sv[RED_WAVES][RED_CAP];
@@ -996,12 +1017,20 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_heavy(
                 }
                 bool lrc = false;
                 int64_t m = bucket_reduce_sorted(Kc, Vc, n, &lrc);
-                if (lrc) m = n;  // left sorted, uncompacted, for k_bucket_long
-                for (int64_t t = 0; t < m; t++) {
-                    KV r;
-                    r.key = Kc[t];
-                    r.val = Vc[t];
-                    red[base + s + t] = r;
+                if (lrc) {
+                    m = n;  // left sorted, uncompacted, for k_bucket_long
+                    for (int64_t t = 0; t < m; t++) {
+                        KV r;
+                        r.key = Kc[t];
+                        r.val = Vc[t];
+                        red[base + s + t] = r;
+                    }
+                } else {
+                    int32_t* hist_out =
+                        hist_out_all +
+                        pcs[__builtin_amdgcn_readfirstlane(pair_of_group[grp])].out_gbase;
+                    for (int64_t t = 0; t < m; t++)
+                        put_reduced(red + base + s + t, Kc[t], Vc[t], hist_out);
                 }
                 nnz_row[i] = (int32_t)m;
                 if (lrc) long_list[atomicAdd(n_long, 1)] = (int32_t)i;
@@ -1023,7 +1052,10 @@ __global__ void k_bucket_long(const int64_t* __restrict__ cell_off, KV* __restri
                               const double* __restrict__ wi,
                               const int32_t* __restrict__ long_list,
                               const int32_t* __restrict__ n_long,
-                              int32_t* __restrict__ nnz_row)
+                              int32_t* __restrict__ nnz_row,
+                              const PairCells* __restrict__ pcs,
+                              const int32_t* __restrict__ pair_of_group,
+                              int32_t* __restrict__ hist_out_all)
 {
     const int32_t nl = *n_long;
     for (int32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nl;
@@ -1037,15 +1069,20 @@ __global__ void k_bucket_long(const int64_t* __restrict__ cell_off, KV* __restri
             for (int64_t t = 0; t < n; t++) r[t].val = weight_of(r[t].val, vol, wi, i);
             sort_records(r, n);
         }
-        nnz_row[i] = (int32_t)bucket_reduce_long(r, n);
+        const int64_t m = bucket_reduce_long(r, n);
+        nnz_row[i] = (int32_t)m;
+        int32_t* hist_out = hist_out_all + pcs[pair_of_group[i >> 6]].out_gbase;
+        for (int64_t t = 0; t < m; t++) put_reduced(r + t, r[t].key, r[t].val, hist_out);
     }
 }
 
-// CSR by input from the reduced groups: a packed group is one contiguous
-// copy; a fallback group copies bucket by bucket.  Also counts the entries
-// of every output cell for the transpose, and writes each pair's int32 row
-// pointers (relative to the pair's first entry; pair k's n_in + 1 pointers
-// start at rowptr_all[cell_gbase + k]).
+// Both CSRs from the reduced groups.  By input: a packed group is one
+// contiguous copy, an unpacked group copies cell by cell.  By output: each
+// entry goes to its output row at the row's offset plus its arrival rank
+// (counted by the reductions); k_rows_sort then orders each row by column,
+// which makes the result deterministic.  Also writes each pair's int32 row
+// pointers by input (relative to the pair's first entry; pair k's n_in + 1
+// pointers start at rowptr_all[cell_gbase + k]).
 __global__ void k_csr_groups(const PairCells* __restrict__ pcs,
                              const int32_t* __restrict__ pair_of_group,
                              int64_t n_groups,
@@ -1056,7 +1093,9 @@ __global__ void k_csr_groups(const PairCells* __restrict__ pcs,
                              int32_t* __restrict__ rowptr_all,
                              int32_t* __restrict__ col,
                              double* __restrict__ out_val,
-                             int32_t* __restrict__ hist_out)
+                             const int64_t* __restrict__ t_off,
+                             int32_t* __restrict__ t_col,
+                             double* __restrict__ t_val)
 {
     const int64_t grp = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
@@ -1071,56 +1110,45 @@ __global__ void k_csr_groups(const PairCells* __restrict__ pcs,
     if (i < i1) rowptr[i - P.cell_gbase] = (int32_t)(row_off[i] - ebase);
     if (i == iend - 1) rowptr[P.n_in] = (int32_t)(row_off[iend] - ebase);
     if (i0 >= i1) return;
-    int32_t* hist = hist_out + P.out_gbase;
+    const int64_t* toff = t_off + P.out_gbase;
     const int64_t d0 = row_off[i0];
     if (group_mode[grp] == 0) {
         const int64_t b0 = cell_off[i0], cnt = row_off[i1] - d0;
-        for (int64_t q = lane; q < cnt; q += 64) {
-            const KV r = rec[b0 + q];
-            const int32_t o = (int32_t)r.key;
-            col[d0 + q] = o;
-            out_val[d0 + q] = r.val;
-            atomicAdd(&hist[o], 1);
+        // the row of entry q: the lanes' rows bracket the group's entries
+        const int64_t my_d = i < i1 ? row_off[i] - d0 : INT64_MAX;
+        for (int64_t q0 = 0; q0 < cnt; q0 += 64) {  // uniform trip count: shuffles below
+            const int64_t q = q0 + lane;
+            // row of entry q = the last lane whose first entry is at or before q
+            int32_t row = 0;
+#pragma unroll
+            for (int b = 32; b > 0; b >>= 1) {
+                const int64_t dd = __shfl(my_d, (row + b) & 63, 64);
+                if (row + b < 64 && dd <= q) row += b;
+            }
+            if (q < cnt) {
+                const KV r = rec[b0 + q];
+                const int32_t o = (int32_t)(uint32_t)r.key;
+                col[d0 + q] = o;
+                out_val[d0 + q] = r.val;
+                const int64_t pos = toff[o] + (int64_t)(r.key >> 32);
+                t_col[pos] = (int32_t)(i0 + row - P.cell_gbase);
+                t_val[pos] = r.val;
+            }
         }
     } else if (i < i1) {
         const int64_t sb = cell_off[i], d = row_off[i], m = row_off[i + 1] - d;
         for (int64_t q = 0; q < m; q++) {
             const KV r = rec[sb + q];
-            const int32_t o = (int32_t)r.key;
+            const int32_t o = (int32_t)(uint32_t)r.key;
             col[d + q] = o;
             out_val[d + q] = r.val;
-            atomicAdd(&hist[o], 1);
+            const int64_t pos = toff[o] + (int64_t)(r.key >> 32);
+            t_col[pos] = (int32_t)(i - P.cell_gbase);
+            t_val[pos] = r.val;
         }
     }
 }
 
-// Transpose scatter: entry q of input row i goes to output row col[q] of the
-// transposed CSR at an atomically claimed slot; k_rows_sort then orders each
-// transposed row by column, which makes the result deterministic.
-__global__ void k_transpose_scatter(const PairCells* __restrict__ pcs,
-                                    const int32_t* __restrict__ pair_of_group,
-                                    int64_t nrows_all,
-                                    const int64_t* __restrict__ row_off,
-                                    const int32_t* __restrict__ col,
-                                    const double* __restrict__ val,
-                                    const int64_t* __restrict__ t_off,
-                                    int32_t* __restrict__ cursor,
-                                    int32_t* __restrict__ t_col,
-                                    double* __restrict__ t_val)
-{
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= nrows_all) return;
-    const PairCells& P = pcs[__builtin_amdgcn_readfirstlane(pair_of_group[i >> 6])];
-    const int32_t li = (int32_t)(i - P.cell_gbase);
-    for (int64_t q = row_off[i]; q < row_off[i + 1]; q++) {
-        const int64_t o = P.out_gbase + col[q];
-        const int64_t pos = t_off[o] + atomicAdd(&cursor[o], 1);
-        if (pos < t_off[o + 1]) {
-            t_col[pos] = li;
-            t_val[pos] = val[q];
-        }
-    }
-}
 
 // Orders every row of the transposed CSR by column (the rows were filled
 // in atomic order).  One wavefront per 64 consecutive rows, whose entries
@@ -1653,19 +1681,24 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
     RG_HIP(hipGetLastError());
     RG_TRY(scan_i32_to_i64(group_mode.p, group_off.p, n_groups, s));
     tm.mark();  // 7
-    DevBuf<int32_t> heavy_list;
+    DevBuf<int32_t> heavy_list, hist_out;
     RG_TRY(heavy_list.alloc(n_groups, s));
+    // entries per output row (the reductions also keep each entry's rank)
+    RG_TRY(hist_out.alloc(out_tot, s));
+    RG_TRY(fill_zero(hist_out.p, out_tot, s));
     k_reduce_groups<<<(unsigned)((n_groups + RED_WAVES - 1) / RED_WAVES),
                       RED_WAVES * 64, 0, s>>>(bucket_off.p, n_groups, group_off.p, rec.p,
                                               rcell.p, red.p, cell_off.p, vol.p, wi_all.p,
                                               nnz_row.p, group_mode.p, long_list.p,
-                                              counters.p + 3, heavy_list.p, counters.p + 5);
+                                              counters.p + 3, heavy_list.p, counters.p + 5,
+                                              d_pcs, pair_of_ing.p, hist_out.p);
     k_reduce_heavy<<<1024, RED_WAVES * 64, 0, s>>>(
         bucket_off.p, n_groups, group_off.p, rec.p, rcell.p, red.p, cell_off.p, vol.p,
         wi_all.p, nnz_row.p, group_mode.p, long_list.p, counters.p + 3, heavy_list.p,
-        counters.p + 5);
+        counters.p + 5, d_pcs, pair_of_ing.p, hist_out.p);
     k_bucket_long<<<256, 64, 0, s>>>(cell_off.p, red.p, vol.p, wi_all.p, long_list.p,
-                                   counters.p + 3, nnz_row.p);
+                                   counters.p + 3, nnz_row.p, d_pcs, pair_of_ing.p,
+                                   hist_out.p);
     RG_HIP(hipGetLastError());
     tm.mark();  // 8
     RG_TRY(scan_i32_to_i64(nnz_row.p, row_off.p, cell_tot, s));
@@ -1689,7 +1722,7 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
             return fail(RG_ERR_ARGUMENT, "more than 2^31 - 1 weights in one element");
 
     // ---- CSR by input, transpose -> CSR by output
-    DevBuf<int32_t> rp_in, c_in, rp_out, c_out, hist_out;
+    DevBuf<int32_t> rp_in, c_in, rp_out, c_out;
     DevBuf<double> v_in, v_out;
     DevBuf<int64_t> off_out;
     RG_TRY(rp_in.alloc(cell_tot + npairs, s));
@@ -1698,19 +1731,13 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
     RG_TRY(rp_out.alloc(out_tot + npairs, s));
     RG_TRY(c_out.alloc(nnz, s));
     RG_TRY(v_out.alloc(nnz, s));
-    RG_TRY(hist_out.alloc(out_tot, s));
     RG_TRY(off_out.alloc(out_tot + 1, s));
-    RG_TRY(fill_zero(hist_out.p, out_tot, s));
     rec.reset();
+    RG_TRY(scan_i32_to_i64(hist_out.p, off_out.p, out_tot, s));
     k_csr_groups<<<grid_for(n_groups * 64, 256), 256, 0, s>>>(
         d_pcs, pair_of_ing.p, n_groups, cell_off.p, row_off.p, group_mode.p,
-        red.p, rp_in.p, c_in.p, v_in.p, hist_out.p);
+        red.p, rp_in.p, c_in.p, v_in.p, off_out.p, c_out.p, v_out.p);
     RG_HIP(hipGetLastError());
-    RG_TRY(scan_i32_to_i64(hist_out.p, off_out.p, out_tot, s));
-    RG_TRY(fill_zero(hist_out.p, out_tot, s));  // cursor
-    k_transpose_scatter<<<grid_for(cell_tot, 128), 128, 0, s>>>(
-        d_pcs, pair_of_ing.p, cell_tot, row_off.p, c_in.p, v_in.p, off_out.p,
-        hist_out.p, c_out.p, v_out.p);
     const int64_t sort_groups = out_tot / 64;
     k_rows_sort<<<(unsigned)((sort_groups + SORT_WAVES - 1) / SORT_WAVES),
                   SORT_WAVES * 64, 0, s>>>(d_pcs, pair_of_outg.p, off_out.p, out_tot,
