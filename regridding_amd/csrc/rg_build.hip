@@ -745,7 +745,7 @@ __device__ inline void sort_cell_in_registers(uint64_t* K, double* V, int32_t n,
             v[j] = weight_of(V[j], vol, wi, i);
         }
     }
-    sort_network<N>(k, v);
+    sort_network<N, uint64_t>(k, v);
 #pragma unroll
     for (int j = 0; j < N; j++)
         if (j < n) {
@@ -1159,6 +1159,31 @@ __global__ void k_csr_groups(const PairCells* __restrict__ pcs,
 constexpr int SORT_WAVES = 2;
 constexpr int SORT_CAP = 1024;  // entries per wavefront (12 KiB)
 
+// One transposed row (columns, values) sorted in registers through a
+// network of N, written back in place.
+template <int N>
+__device__ inline void sort_row_in_registers(int32_t* C, double* V, int32_t n)
+{
+    int32_t k[N];
+    double v[N];
+#pragma unroll
+    for (int j = 0; j < N; j++) {
+        k[j] = INT32_MAX;  // padding sorts last (columns are below it)
+        v[j] = 0;
+        if (j < n) {
+            k[j] = C[j];
+            v[j] = V[j];
+        }
+    }
+    sort_network<N, int32_t>(k, v);
+#pragma unroll
+    for (int j = 0; j < N; j++)
+        if (j < n) {
+            C[j] = k[j];
+            V[j] = v[j];
+        }
+}
+
 __global__ __launch_bounds__(SORT_WAVES * 64) void k_rows_sort(
     const PairCells* __restrict__ pcs, const int32_t* __restrict__ pair_of_group,
     const int64_t* __restrict__ off, int64_t nrows_all, int32_t* __restrict__ col,
@@ -1194,7 +1219,14 @@ __global__ __launch_bounds__(SORT_WAVES * 64) void k_rows_sort(
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (r < r1) sort_pairs(C + s, V + s, n);
+        if (r < r1) {
+            if (n <= 8)
+                sort_row_in_registers<8>(C + s, V + s, (int32_t)n);
+            else if (n <= 16)
+                sort_row_in_registers<16>(C + s, V + s, (int32_t)n);
+            else
+                sort_pairs(C + s, V + s, n);
+        }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");

@@ -740,8 +740,8 @@ RG_HD int64_t bucket_reduce(uint64_t* keys, double* vals, int64_t n,
 
 // Bitonic sorting network over N keys in registers (N a power of two, every
 // index static once unrolled), values following.
-template <int N>
-RG_HD inline void sort_network(uint64_t* k, double* v)
+template <int N, class K>
+RG_HD inline void sort_network(K* k, double* v)
 {
 #pragma unroll
     for (int size = 2; size <= N; size <<= 1) {
@@ -753,7 +753,7 @@ RG_HD inline void sort_network(uint64_t* k, double* v)
                 if (j > i) {
                     const bool up = (i & size) == 0;
                     if ((k[i] > k[j]) == up) {
-                        const uint64_t tk = k[i];
+                        const K tk = k[i];
                         k[i] = k[j];
                         k[j] = tk;
                         const double tv = v[i];
