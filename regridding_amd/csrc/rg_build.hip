@@ -620,7 +620,7 @@ __device__ inline void wave_sync_lds()
 // bucket's records as one contiguous run.  A full table falls back to one
 // claim per triple.  A chunk's triples beyond the staging capacity come
 // from the overflow buffer (k_restage).
-constexpr int SCAT_R = 8;     // triples per lane per round
+constexpr int SCAT_R = 16;    // triples per lane per round
 constexpr int SCAT_HT = 256;  // hash entries per wavefront
 
 struct ScatTable {
