@@ -1373,9 +1373,8 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
         if (d.nxi < 2 || d.nyi < 2 || d.nxo < 2 || d.nyo < 2)
             return fail(RG_ERR_ARGUMENT,
                         "each grid needs at least 2 vertices along both axes" + at);
-        if ((d.nxi - 1) * (d.nyi - 1) >= INT32_MAX ||
-            (d.nxo - 1) * (d.nyo - 1) >= INT32_MAX)
-            return fail(RG_ERR_ARGUMENT, "more than 2^31 - 1 cells in one grid" + at);
+        if (d.nxi * d.nyi >= INT32_MAX || d.nxo * d.nyo >= INT32_MAX)
+            return fail(RG_ERR_ARGUMENT, "more than 2^31 - 1 vertices in one grid" + at);
     }
     int64_t K = 16, stage_per_edge = 8;
     int block = 128;

@@ -311,25 +311,31 @@ constexpr int32_t STATE_FAILED = -2;   // step budget exceeded
 // quotient by the input cell's area is the weight.  Returns the state at
 // vertex k1.
 template <class Sink>
-RG_HD int32_t walk_chunk(const Build& B, const Pass& P, int64_t row,
+RG_HD int32_t walk_chunk(const Build& B, const Pass& P, int64_t row64,
                          int64_t k0, int64_t k1, int32_t start, Sink& sink)
 {
+    // Indices are 32-bit: a build requires fewer than 2^31 vertices per grid
+    // (so every vertex, cell and sweep index below fits).
     const StaticGrid& s = P.st;
-    const int64_t ncx = s.nx - 1, ncy = s.ny - 1;
+    const int32_t ncx = (int32_t)(s.nx - 1), ncy = (int32_t)(s.ny - 1), nys = (int32_t)s.ny;
+    const int32_t row = (int32_t)row64, kend = (int32_t)k1;
+    const int32_t ncy_sweep = (int32_t)P.ncy_sweep, nrows = (int32_t)P.nrows;
+    const int32_t sk = (int32_t)P.sk;
     const int64_t limit = step_limit(s, k1 - k0);
-    double x1 = sweep_x(P, row, k0), y1 = sweep_y(P, row, k0);
-    int64_t k = k0;
+    int32_t k = (int32_t)k0;
+    int32_t widx = row * (int32_t)P.sr + k * sk;  // sweep vertex (row, k)
+    double x1 = P.xs[widx], y1 = P.ys[widx];
     bool outside = start < 0;
-    int64_t sx = outside ? 0 : (int64_t)start / ncy;
-    int64_t sy = outside ? 0 : (int64_t)start % ncy;
-    int64_t edge_last = -1;  // sys.maxsize
+    int32_t sx = outside ? 0 : start / ncy;
+    int32_t sy = outside ? 0 : start % ncy;
+    int32_t edge_last = -1;  // sys.maxsize
     int64_t steps = 0;
-    int64_t csx = -2, csy = -2;  // cell whose corners are cached
+    int32_t csx = -2, csy = -2;  // cell whose corners are cached
     double2 c0, c1, c2, c3;
     c0.x = c0.y = c1.x = c1.y = c2.x = c2.y = c3.x = c3.y = 0;
-    while (k < k1) {
+    while (k < kend) {
         if (++steps > limit) return STATE_FAILED;
-        double x2 = sweep_x(P, row, k + 1), y2 = sweep_y(P, row, k + 1);
+        double x2 = P.xs[widx + sk], y2 = P.ys[widx + sk];
         if (outside) {
             int64_t cell;
             int code;
@@ -337,12 +343,13 @@ RG_HD int32_t walk_chunk(const Build& B, const Pass& P, int64_t row,
             if (outside_scan(s, x1, y1, x2, y2, edge_last, &cell, &code, &t)) {
                 x2 = x1 + t * (x2 - x1);
                 y2 = y1 + t * (y2 - y1);
-                sx = cell / ncy;
-                sy = cell % ncy;
+                sx = (int32_t)cell / ncy;
+                sy = (int32_t)cell % ncy;
                 edge_last = code;
                 outside = false;
             } else {
                 k += 1;
+                widx += sk;
                 edge_last = -1;
             }
         } else {
@@ -352,7 +359,7 @@ RG_HD int32_t walk_chunk(const Build& B, const Pass& P, int64_t row,
             // registers while the walk stays in the cell; a move to a
             // neighbour loads only the two new corners.
             if (sx != csx || sy != csy) {
-                const int64_t a = sx * s.ny + sy, b = a + s.ny;
+                const int32_t a = sx * nys + sy, b = a + nys;
                 if (sx == csx + 1 && sy == csy) {
                     c0 = c1; c3 = c2; c1 = s.xy[b]; c2 = s.xy[b + 1];
                 } else if (sx == csx - 1 && sy == csy) {
@@ -388,8 +395,8 @@ RG_HD int32_t walk_chunk(const Build& B, const Pass& P, int64_t row,
                 r = seg_params(x1, y1, x2, y2, cx2, cy2, cx3, cy3);
                 if (seg_hit(r)) hit = 3;
             }
-            int64_t nsx = sx, nsy = sy, k_next = k;
-            int64_t e_next;
+            int32_t nsx = sx, nsy = sy, k_next = k;
+            int32_t e_next;
             if (hit >= 0) {
                 x2 = x1 + r.t * (x2 - x1);
                 y2 = y1 + r.t * (y2 - y1);
@@ -404,15 +411,14 @@ RG_HD int32_t walk_chunk(const Build& B, const Pass& P, int64_t row,
             // index (row, k) and static cell (sx, sy).
             double area = area_tri(x1, y1, x2, y2);
             if (P.axis == 0) area = -area;
-            const int64_t cell_static = sx * ncy + sy;
+            const int32_t cell_static = sx * ncy + sy;
             for (int side = 0; side < 2; side++) {
-                int64_t rr = side == 0 ? row - 1 : row;
+                const int32_t rr = side == 0 ? row - 1 : row;
                 if (side == 0 && !(rr >= 0)) continue;
-                if (side == 1 && !(rr < P.nrows - 1)) continue;
-                int64_t cell_sweep =
-                    P.axis == 0 ? k * P.ncy_sweep + rr : rr * P.ncy_sweep + k;
-                int64_t cin = P.sweep_input ? cell_sweep : cell_static;
-                int64_t cout = P.sweep_input ? cell_static : cell_sweep;
+                if (side == 1 && !(rr < nrows - 1)) continue;
+                const int32_t cell_sweep = P.axis == 0 ? k * ncy_sweep + rr : rr * ncy_sweep + k;
+                const int32_t cin = P.sweep_input ? cell_sweep : cell_static;
+                const int32_t cout = P.sweep_input ? cell_static : cell_sweep;
                 // the weight is term / area(cin) [* weights_input[cin]]
                 // (:797-802, :820-825); the reduction divides
                 sink.emit(cin, cout, side == 0 ? area : -area);
@@ -424,13 +430,14 @@ RG_HD int32_t walk_chunk(const Build& B, const Pass& P, int64_t row,
                 sx = nsx;
                 sy = nsy;
             }
+            if (k_next != k) widx += sk;
             k = k_next;
             edge_last = e_next;
         }
         x1 = x2;
         y1 = y2;
     }
-    return outside ? STATE_OUTSIDE : (int32_t)(sx * ncy + sy);
+    return outside ? STATE_OUTSIDE : sx * ncy + sy;
 }
 
 // Guessed walk state at the first vertex of chunk c > 0 of a row: the
