@@ -31,7 +31,9 @@ enum rg_status {
     RG_ERR_MEMORY = 3,    /* device allocation failed */
     RG_ERR_GEOMETRY = 4,  /* the sweep did not terminate (exactly degenerate
                              corner crossings, where the reference loops) */
-    RG_ERR_INJECTED = 5   /* failure forced through rg_inject_failure */
+    RG_ERR_INJECTED = 5,  /* failure forced through rg_inject_failure */
+    RG_ERR_CAPACITY = 6   /* a captured graph's replay exceeded its fixed
+                             capacities (rg_graph_status) */
 };
 
 /* Opaque device-resident weights of one 2-D element. */
@@ -301,6 +303,47 @@ int rg_multilinear_1d(const double* x_input, int64_t n_input, const double* x_ou
  * named site ("build", "apply", "alloc") fail with RG_ERR_INJECTED.  NULL
  * clears pending injections. */
 int rg_inject_failure(const char* site);
+
+/* ---- one hipGraph for a batched build and apply ---------------------
+ *
+ * No counterpart in the reference: the batched build (above) followed by
+ * the FP64 apply of one job per pair, captured once into a hipGraph and
+ * replayed with one launch.  A replay reads the coordinates and values at
+ * the same device addresses as at creation (their contents may change;
+ * the shapes may not) and writes the outputs there.  rg_graph_create builds
+ * once to measure the data-dependent sizes, sizes a workspace with `slack`
+ * (0.25: a quarter more triples and weights than measured), then captures.
+ * A replay whose data exceed a capacity (or whose sweep fails) leaves
+ * incomplete results and sets rg_graph_status's flags: rebuild with more
+ * slack.  The graph's buffers are its own: build handles with
+ * rg_graph_weights (a copy of pair k's weights from the last replay;
+ * RG_ERR_CAPACITY when that replay set flags).  Status and weights need a
+ * launch first.  Replays on one graph share its buffers: they must not
+ * overlap (launch them on one stream). */
+typedef struct rg_graph* rg_graph_t;
+
+typedef struct rg_graph_job {
+    const double* values_input; /* (batch, n_in) of the pair, device */
+    double* values_output;      /* (batch, n_out), device */
+    int64_t batch;              /* 0: no apply for this pair */
+} rg_graph_job;
+
+/* flags of rg_graph_status */
+#define RG_GRAPH_TRIPLES_CAPACITY 1
+#define RG_GRAPH_OVERFLOW_CAPACITY 2
+#define RG_GRAPH_WEIGHTS_CAPACITY 4
+#define RG_GRAPH_SWEEP_FAILED 8
+#define RG_GRAPH_MISMATCH 16
+
+int rg_graph_create(const rg_pair_desc* pairs, int64_t npairs, const rg_tuning* tuning,
+                    const rg_graph_job* jobs, double slack, rg_graph_t* out, void* stream);
+int rg_graph_launch(rg_graph_t g, void* stream);
+/* waits for `stream` and reads the flags of the last replay (0: complete) */
+int rg_graph_status(rg_graph_t g, int32_t* flags, void* stream);
+int rg_graph_weights(rg_graph_t g, int64_t k, rg_weights_t* out, void* stream);
+int rg_graph_capacity(rg_graph_t g, int64_t* triples, int64_t* overflow, int64_t* weights,
+                      int64_t* workspace_bytes);
+int rg_graph_destroy(rg_graph_t g);
 
 /* Replaces _fill/_gauss_seidel.py:83-139 (_fill_gauss_seidel_2d): num_iterations
  * red-black Gauss-Seidel sweeps, in place, over T elements of (ny, nx) values

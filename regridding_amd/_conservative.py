@@ -111,6 +111,7 @@ class PackedPairs:
                 pos += a.size
             offsets.append(offs)
         self.buffer = DeviceArray.from_host(flat)
+        self._layout = (shapes, offsets, flat.size)
         self.n = len(arrays)
         self.desc = (rg_pair_desc * max(self.n, 1))()
         base = self.buffer.ptr
@@ -121,6 +122,29 @@ class PackedPairs:
             d.xo, d.yo = base + 8 * offs[2], base + 8 * offs[3]
             d.nxo, d.nyo = so
             d.wi = None if offs[4] is None else base + 8 * offs[4]
+
+
+    def update(self, pairs) -> None:
+        """Overwrite the coordinates (and weightings) in place with pairs of
+        the same shapes: builds reading this buffer see the new values."""
+        shapes, offsets, size = self._layout
+        flat = np.empty(max(size, 1), dtype=np.float64)
+        if len(pairs) != len(shapes):
+            raise ValueError("a different number of pairs")
+        for item, (si, so), offs in zip(pairs, shapes, offsets):
+            gi, go = item[0], item[1]
+            wi = item[2] if len(item) > 2 else None
+            arrs = (gi[0], gi[1], go[0], go[1], wi)
+            for a, o, sh in zip(arrs, offs, (si, si, so, so, None)):
+                if o is None:
+                    if a is not None:
+                        raise ValueError("a weighting where the packed pair has none")
+                    continue
+                a = np.ascontiguousarray(a, dtype=np.float64)
+                if sh is not None and a.shape != tuple(sh):
+                    raise ValueError(f"shape {a.shape} differs from the packed {tuple(sh)}")
+                flat[o: o + a.size] = a.ravel()
+        self.buffer.copy_from_host(flat[: max(size, 1)] if size else flat)
 
 
 def pack_pairs(pairs) -> PackedPairs:

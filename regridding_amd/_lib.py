@@ -186,6 +186,12 @@ SIGNATURES = {
     "rg_conservative_1d_emit": (ctypes.c_int, [_vp, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp,
                                                _vp, _vp, _vp]),
     "rg_cell_length_1d": (ctypes.c_int, [_vp, _i64, _i64, _vp, _vp]),
+    "rg_graph_create": (ctypes.c_int, [_vp, _i64, _vp, _vp, ctypes.c_double, _vp, _vp]),
+    "rg_graph_launch": (ctypes.c_int, [_vp, _vp]),
+    "rg_graph_status": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32), _vp]),
+    "rg_graph_weights": (ctypes.c_int, [_vp, _i64, _vp, _vp]),
+    "rg_graph_capacity": (ctypes.c_int, [_vp, _pi64, _pi64, _pi64, _pi64]),
+    "rg_graph_destroy": (ctypes.c_int, [_vp]),
     "rg_fill_gauss_seidel_2d": (ctypes.c_int, [_vp, _vp, _i64, _i64, _i64, _i64, _vp]),
     "rg_interp_linear_1d": (ctypes.c_int, [_vp, _i64, _vp, _i64, _i64, _vp, _vp]),
     "rg_interp_linear_2d": (ctypes.c_int, [_vp, _i64, _i64, _vp, _vp, _i64, _i64, _vp, _vp]),
@@ -264,6 +270,14 @@ def _finalize_ptr(ptr: int) -> None:
         _lib.rg_free(ptr)
 
 
+class rg_graph_job(ctypes.Structure):
+    _fields_ = [
+        ("values_input", ctypes.c_void_p),
+        ("values_output", ctypes.c_void_p),
+        ("batch", ctypes.c_int64),
+    ]
+
+
 class DeviceArray:
     """A block of device memory owned by the library's caching allocator."""
 
@@ -289,6 +303,14 @@ class DeviceArray:
         if a.nbytes:
             check(lib().rg_memcpy_htod(d.ptr, a.ctypes.data, a.nbytes, stream))
         return d
+
+    def copy_from_host(self, a: np.ndarray, stream: int = 0) -> None:
+        """Overwrite the block with `a` (same byte size)."""
+        a = np.ascontiguousarray(a)
+        if a.nbytes != self.nbytes:
+            raise ValueError(f"{a.nbytes} bytes do not fit a block of {self.nbytes}")
+        if a.nbytes:
+            check(lib().rg_memcpy_htod(self.ptr, a.ctypes.data, a.nbytes, stream))
 
     def to_host(self, stream: int = 0, out: np.ndarray | None = None) -> np.ndarray:
         if out is None:

@@ -407,24 +407,12 @@ __global__ void k_to_f32(const double* __restrict__ a, float* __restrict__ b,
 // Many applies, each with its own weights, in one launch.  Jobs are cut into
 // blocks of APPLY_BLOCK output rows x DB images; block b finds its job in the
 // jobs' block prefix.  The accumulation per output is the one of k_apply.
-struct JobDesc {
-    const int32_t* rowptr;
-    const int32_t* col;
-    const void* w;
-    const void* x;
-    void* y;
-    int64_t nrows, ncols, batch;
-    int64_t block_base;  // first block of this job
-    int64_t nrt;         // row tiles of the job
-};
-
-constexpr int JOB_DB = 8;
-
 template <class T>
 __global__ __launch_bounds__(APPLY_BLOCK) void k_apply_jobs(
     const JobDesc* __restrict__ jobs, const int64_t* __restrict__ block_prefix,
-    int njobs)
+    int njobs, const int32_t* __restrict__ abort_flag)
 {
+    if (abort_flag && *abort_flag) return;  // uniform: one word for the grid
     const int64_t b = blockIdx.x;
     const JobDesc& J = jobs[seg_of(block_prefix, njobs, b)];
     const int64_t lt = b - J.block_base;
@@ -703,15 +691,28 @@ int launch_jobs_common(const rg_apply_job* jobs, int64_t njobs, bool f32,
     const int64_t* d_prefix = (const int64_t*)(dt.p + poff);
     if (f32)
         k_apply_jobs<float><<<(unsigned)nblocks, APPLY_BLOCK, 0, s>>>(
-            d_jobs, d_prefix, (int)table.size());
+            d_jobs, d_prefix, (int)table.size(), nullptr);
     else
         k_apply_jobs<double><<<(unsigned)nblocks, APPLY_BLOCK, 0, s>>>(
-            d_jobs, d_prefix, (int)table.size());
+            d_jobs, d_prefix, (int)table.size(), nullptr);
     RG_HIP(hipGetLastError());
     return RG_OK;  // dt returns to the stream's pool after the kernel
 }
 
 }  // namespace
+
+int apply_jobs_device(const JobDesc* jobs, const int64_t* prefix, int njobs, int64_t nblocks,
+                      const int32_t* abort_flag, hipStream_t s)
+{
+    if (njobs == 0 || nblocks == 0) return RG_OK;
+    if (nblocks >= INT32_MAX) return fail(RG_ERR_ARGUMENT, "batched apply too large");
+    k_apply_jobs<double><<<(unsigned)nblocks, APPLY_BLOCK, 0, s>>>(jobs, prefix, njobs,
+                                                                  abort_flag);
+    RG_HIP(hipGetLastError());
+    return RG_OK;
+}
+
+int64_t apply_block_rows() { return APPLY_BLOCK; }
 
 }  // namespace rg
 

@@ -136,9 +136,10 @@ struct OverflowSink {
     int32_t n;
     int64_t cap;
     Triple* __restrict__ out;
+    int64_t room;  // triples the chunk's range can take
     __device__ void emit(int64_t cin, int64_t cout, double a)
     {
-        if (n >= cap) {
+        if (n >= cap && n - cap < room) {
             Triple t;
             t.cin = (int32_t)cin;
             t.cout = (int32_t)cout;
@@ -585,7 +586,7 @@ __global__ __launch_bounds__(WALK_MAX_BLOCK) void k_restage(const Build* __restr
                           const int32_t* __restrict__ start,
                           const int32_t* __restrict__ count,
                           const int64_t* __restrict__ ov_off,
-                          Triple* __restrict__ overflow, int64_t cap,
+                          Triple* __restrict__ overflow, int64_t cap, int64_t ov_cap,
                           int32_t* __restrict__ restaged)
 {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -598,7 +599,10 @@ __global__ __launch_bounds__(WALK_MAX_BLOCK) void k_restage(const Build* __restr
     const Pass& P = B.pass[r.p];
     int64_t k0, k1;
     chunk_range(B, P, r.c, &k0, &k1);
-    OverflowSink sink{0, cap, overflow + ov_off[g]};
+    // (a capacity-mode build sizes the buffer in advance: a chunk past its
+    // end keeps nothing, and the build's flag reports it)
+    const int64_t room = ov_cap - ov_off[g];
+    OverflowSink sink{0, cap, overflow + ov_off[g], room > 0 ? room : 0};
     (void)walk_chunk(B, P, r.row, k0, k1, start[g], sink);
     atomicAdd(restaged, 1);
 }
@@ -635,7 +639,7 @@ __global__ __launch_bounds__(WALK_MAX_BLOCK) void k_scatter(const Build* __restr
                           const int32_t* __restrict__ count,
                           const Triple* __restrict__ stage, int64_t cap,
                           const Triple* __restrict__ ovbuf,
-                          const int64_t* __restrict__ ov_off,
+                          const int64_t* __restrict__ ov_off, int64_t ov_cap,
                           const int64_t* __restrict__ chunk_off,
                           uint32_t* __restrict__ cursor_all,
                           KV* __restrict__ rec, uint8_t* __restrict__ rcell,
@@ -652,9 +656,13 @@ __global__ __launch_bounds__(WALK_MAX_BLOCK) void k_scatter(const Build* __restr
     const int64_t g = B.chunk_gbase + r.g;
     // every lane of the wave sweeps one pass: the bucket row is wave-uniform
     uint32_t* cursor = cursor_all + (B.cell_gbase >> 6) + (int64_t)r.p * ngroups_all;
-    const int32_t n = r.valid ? count[g] : 0;
+    int32_t n = r.valid ? count[g] : 0;
     const uint32_t ord = r.valid ? (uint32_t)chunk_off[g] : 0;
     const Triple* ov = n > cap ? ovbuf + ov_off[g] : nullptr;
+    if (n > cap) {  // only what the overflow buffer holds (all of it unless capped)
+        const int64_t room = ov_cap - ov_off[g];
+        if ((int64_t)n - cap > room) n = (int32_t)(cap + (room > 0 ? room : 0));
+    }
     int32_t mx = n;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
@@ -724,6 +732,11 @@ __global__ __launch_bounds__(WALK_MAX_BLOCK) void k_scatter(const Build* __restr
 // consecutive cells, each cell reduced in place at its own offset (the
 // group is then marked unpacked); a single cell beyond the share is left to
 // k_bucket_long.
+// Bits of the flag word (counters[8]) of a capacity-mode build (graph
+// capture): which fixed capacity the data exceeded, or that the build failed.
+constexpr int32_t CAP_TRIPLES = 1, CAP_OVERFLOW = 2, CAP_WEIGHTS = 4, CAP_FAILED = 8,
+                  CAP_MISMATCH = 16;
+
 constexpr int RED_WAVES = 2;     // wavefronts per block
 constexpr int RED_CAP = 1024;    // LDS entries per wavefront (16 KiB)
 constexpr int RED_NET = 16;      // largest cell sorted in registers
@@ -825,7 +838,8 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
     int32_t* __restrict__ group_mode, int32_t* __restrict__ long_list,
     int32_t* __restrict__ n_long, int32_t* __restrict__ heavy_list,
     int32_t* __restrict__ n_heavy, const PairCells* __restrict__ pcs,
-    const int32_t* __restrict__ pair_of_group, int32_t* __restrict__ hist_out_all)
+    const int32_t* __restrict__ pair_of_group, int32_t* __restrict__ hist_out_all,
+    int64_t red_cap, int32_t* __restrict__ cap_flag)
 {
     __shared__ uint64_t sk[RED_WAVES][RED_CAP];
     __shared__ double sv[RED_WAVES][RED_CAP];
@@ -836,6 +850,17 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
     if (grp >= n_groups) return;  // uniform per wavefront; no block barriers below
     const int64_t base = group_off[grp];
     const int64_t total = group_off[grp + 1] - base;
+    // (the records sit pass-major: the group's last pass range ends last)
+    if (base + total > red_cap || bucket_off[3 * n_groups + grp + 1] > red_cap) {
+        // past the records a capacity-mode build holds: nothing of it is kept
+        nnz_row[grp * 64 + lane] = 0;
+        cell_off[grp * 64 + lane] = 0;
+        if (lane == 0) {
+            group_mode[grp] = 0;
+            atomicOr(cap_flag, CAP_TRIPLES);
+        }
+        return;
+    }
     if (total > RED_CAP) {
         if (lane == 0) heavy_list[atomicAdd(n_heavy, 1)] = (int32_t)grp;
         return;
@@ -1095,7 +1120,7 @@ __global__ void k_csr_groups(const PairCells* __restrict__ pcs,
                              double* __restrict__ out_val,
                              const int64_t* __restrict__ t_off,
                              int32_t* __restrict__ t_col,
-                             double* __restrict__ t_val)
+                             double* __restrict__ t_val, int64_t nnz_cap)
 {
     const int64_t grp = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
@@ -1104,6 +1129,9 @@ __global__ void k_csr_groups(const PairCells* __restrict__ pcs,
     const PairCells& P = pcs[k];
     const int64_t i0 = grp * 64, iend = P.cell_gbase + P.n_in;
     const int64_t i1 = i0 + 64 < iend ? i0 + 64 : iend;
+    // a capacity-mode build past its weight capacity keeps nothing more
+    // (its flag reports it); an ordinary build passes its own weight count
+    if (row_off[i1 > i0 ? i1 : i0] > nnz_cap) return;
     const int64_t ebase = row_off[P.cell_gbase];
     int32_t* rowptr = rowptr_all + P.cell_gbase + k;
     const int64_t i = i0 + lane;
@@ -1131,8 +1159,10 @@ __global__ void k_csr_groups(const PairCells* __restrict__ pcs,
                 col[d0 + q] = o;
                 out_val[d0 + q] = r.val;
                 const int64_t pos = toff[o] + (int64_t)(r.key >> 32);
-                t_col[pos] = (int32_t)(i0 + row - P.cell_gbase);
-                t_val[pos] = r.val;
+                if (pos < nnz_cap) {
+                    t_col[pos] = (int32_t)(i0 + row - P.cell_gbase);
+                    t_val[pos] = r.val;
+                }
             }
         }
     } else if (i < i1) {
@@ -1143,8 +1173,10 @@ __global__ void k_csr_groups(const PairCells* __restrict__ pcs,
             col[d + q] = o;
             out_val[d + q] = r.val;
             const int64_t pos = toff[o] + (int64_t)(r.key >> 32);
-            t_col[pos] = (int32_t)(i - P.cell_gbase);
-            t_val[pos] = r.val;
+            if (pos < nnz_cap) {
+                t_col[pos] = (int32_t)(i - P.cell_gbase);
+                t_val[pos] = r.val;
+            }
         }
     }
 }
@@ -1187,7 +1219,7 @@ __device__ inline void sort_row_in_registers(int32_t* C, double* V, int32_t n)
 __global__ __launch_bounds__(SORT_WAVES * 64) void k_rows_sort(
     const PairCells* __restrict__ pcs, const int32_t* __restrict__ pair_of_group,
     const int64_t* __restrict__ off, int64_t nrows_all, int32_t* __restrict__ col,
-    double* __restrict__ val, int32_t* __restrict__ rowptr_all)
+    double* __restrict__ val, int32_t* __restrict__ rowptr_all, int64_t nnz_cap)
 {
     __shared__ int32_t sc[SORT_WAVES][SORT_CAP];
     __shared__ double sv[SORT_WAVES][SORT_CAP];
@@ -1203,6 +1235,7 @@ __global__ __launch_bounds__(SORT_WAVES * 64) void k_rows_sort(
     if (r < rend) rowptr[r - P.out_gbase] = (int32_t)(off[r] - obase);
     if (r == rend - 1) rowptr[P.n_out] = (int32_t)(off[rend] - obase);
     const int64_t r1 = r0 + 64 < nrows_all ? r0 + 64 : nrows_all;
+    if (off[r1] > nnz_cap) return;  // past a capacity-mode build's capacity
     const int64_t base = off[r0], total = off[r1] - base;
     int64_t s = 0, n = 0;
     if (r < r1) {
@@ -1355,12 +1388,95 @@ size_t append_table(std::vector<unsigned char>& buf, const T* src, size_t n)
 // The build of npairs independent grid pairs: every phase is one launch over
 // the concatenation of all pairs' chunks, rows and (padded) cells.  A single
 // pair is a batch of one.
+// Capacity mode of build_batched, for capture in a hipGraph: every
+// data-dependent size (triples, overflow triples, weights) is a fixed
+// capacity, nothing is read back to the host, kernels that could pass a
+// capacity skip their work and raise a bit in the flag word instead, and
+// the build ends with the FP64 batched apply of one job per pair.
+struct GraphJob {
+    const double* x;  // (batch, n_in) values of the pair's input cells
+    double* y;        // (batch, n_out)
+    int64_t batch;    // 0: no apply for this pair
+};
+
+struct CaptureCaps {
+    // false: an ordinary build that records its totals (T_seen ...) and
+    // makes handles; true: the capacity mode described above
+    bool capacity = false;
+    int64_t T_seen = 0, ov_seen = 0, nnz_seen = 0;
+    int64_t T_cap = 0, ov_cap = 0, nnz_cap = 0;
+    void* meta_pinned = nullptr;  // caller-owned pinned snapshot of the tables
+    size_t meta_pinned_cap = 0;
+    const GraphJob* jobs = nullptr;  // one per pair, or null
+    // filled by build_batched (device memory of the caller's arena)
+    int32_t* counters = nullptr;  // word 8: capacity bits (CAP_*), words 0-7 as in a build
+    int64_t* bases = nullptr;     // per pair: first weight, first triple (slot npairs: totals)
+    int32_t *rp_in = nullptr, *c_in = nullptr, *rp_out = nullptr, *c_out = nullptr;
+    double *v_in = nullptr, *v_out = nullptr;
+    size_t meta_bytes = 0;        // bytes of the table snapshot
+    std::vector<PairCells> pcs;
+    std::vector<int64_t> nchunks;
+};
+
+
+__global__ void k_caps_check_triples(const int64_t* __restrict__ t_chunks,
+                                     const int64_t* __restrict__ t_hist,
+                                     int32_t* __restrict__ counters, int64_t T_cap, int64_t ov_cap)
+{
+    int32_t f = 0;
+    if (*t_chunks > T_cap) f |= CAP_TRIPLES;
+    if (*t_chunks != *t_hist) f |= CAP_MISMATCH;
+    if (counters[2] != 0) f |= CAP_FAILED;
+    int64_t ov;
+    ov = (int64_t)((const unsigned long long*)(counters + 6))[0];
+    if (ov > ov_cap) f |= CAP_OVERFLOW;
+    if (f) atomicOr(&counters[8], f);
+}
+
+__global__ void k_caps_check_weights(const int64_t* __restrict__ nnz_total,
+                                     int32_t* __restrict__ counters, int64_t nnz_cap)
+{
+    int32_t f = 0;
+    if (*nnz_total > nnz_cap) f |= CAP_WEIGHTS;
+    if (counters[2] != 0) f |= CAP_MISMATCH;
+    if (f) atomicOr(&counters[8], f);
+}
+
+// The job table of the graph's apply: each pair's CSR by output sits at its
+// first weight (known on the device only).
+__global__ void k_graph_jobs(JobDesc* __restrict__ jobs, const int32_t* __restrict__ job_pair,
+                             int njobs, const PairCells* __restrict__ pcs,
+                             const int64_t* __restrict__ bases, const int32_t* __restrict__ rp_out,
+                             const int32_t* __restrict__ c_out, const double* __restrict__ v_out)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= njobs) return;
+    const int k = job_pair[j];
+    jobs[j].rowptr = rp_out + pcs[k].out_gbase + k;
+    jobs[j].col = c_out + bases[2 * k];
+    jobs[j].w = v_out + bases[2 * k];
+}
+
+int build_batched(const rg_pair_desc* pairs, int64_t npairs,
+                  const rg_tuning* tuning, rg_weights** out, hipStream_t s,
+                  CaptureCaps* caps);
+
+// The ordinary build (also called from the serving runtime).
 int build_batched(const rg_pair_desc* pairs, int64_t npairs,
                   const rg_tuning* tuning, rg_weights** out, hipStream_t s)
 {
+    return build_batched(pairs, npairs, tuning, out, s, nullptr);
+}
+
+int build_batched(const rg_pair_desc* pairs, int64_t npairs,
+                  const rg_tuning* tuning, rg_weights** out, hipStream_t s,
+                  CaptureCaps* caps)
+{
+    const bool capmode = caps && caps->capacity;
     if (npairs <= 0) return fail(RG_ERR_ARGUMENT, "npairs must be positive");
-    if (!pairs || !out) return fail(RG_ERR_ARGUMENT, "pairs or out is NULL");
-    for (int64_t k = 0; k < npairs; k++) out[k] = nullptr;
+    if (!pairs || (!out && !capmode)) return fail(RG_ERR_ARGUMENT, "pairs or out is NULL");
+    if (out)
+        for (int64_t k = 0; k < npairs; k++) out[k] = nullptr;
     if (injected_failure("build"))
         return fail(RG_ERR_INJECTED, "injected build failure");
     if (npairs >= (int64_t)1 << 24)
@@ -1385,7 +1501,7 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
         if (tuning->block > 0) block = tuning->block;
         if (tuning->stage_per_edge > 0) stage_per_edge = tuning->stage_per_edge;
         flags = tuning->flags;
-        timings = (flags & RG_TUNE_TIMINGS) != 0;
+        timings = (flags & RG_TUNE_TIMINGS) != 0 && !capmode;
     }
     if (block % 64 != 0 || block > WALK_MAX_BLOCK)
         return fail(RG_ERR_ARGUMENT, "block must be a multiple of 64, <= 512");
@@ -1543,14 +1659,53 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
     const size_t o_row = append_table(hb, row_pf.data(), row_pf.size());
     const size_t o_ing = append_table(hb, ing_pf.data(), ing_pf.size());
     const size_t o_outg = append_table(hb, outg_pf.data(), outg_pf.size());
+    // the graph's apply jobs (shapes only; the CSR pointers are set on the
+    // device once the weights exist)
+    std::vector<JobDesc> gjobs;
+    std::vector<int64_t> gprefix;
+    std::vector<int32_t> gpair;
+    int64_t gblocks = 0;
+    if (capmode && caps->jobs) {
+        for (int64_t k = 0; k < npairs; k++) {
+            const GraphJob& J = caps->jobs[k];
+            if (J.batch <= 0 || pcs[k].n_out == 0) continue;
+            JobDesc d = {};
+            d.x = J.x;
+            d.y = J.y;
+            d.nrows = pcs[k].n_out;
+            d.ncols = pcs[k].n_in;
+            d.batch = J.batch;
+            d.nrt = (d.nrows + apply_block_rows() - 1) / apply_block_rows();
+            d.block_base = gblocks;
+            gprefix.push_back(gblocks);
+            gblocks += d.nrt * ((d.batch + JOB_DB - 1) / JOB_DB);
+            gjobs.push_back(d);
+            gpair.push_back((int32_t)k);
+        }
+    }
+    const size_t o_gjobs = append_table(hb, gjobs.data(), gjobs.size());
+    const size_t o_gprefix = append_table(hb, gprefix.data(), gprefix.size());
+    const size_t o_gpair = append_table(hb, gpair.data(), gpair.size());
     DevBuf<unsigned char> meta;
     RG_TRY(meta.alloc(hb.size(), s));
     PinnedUpload up;
     up.s = s;
     void* pinned = nullptr;
-    RG_TRY(PinnedUpload::get(&pinned, hb.size()));
+    if (capmode) {
+        // a snapshot the graph keeps: its copy node re-reads it every replay
+        if (!caps->meta_pinned) {  // first (uncaptured) run: make it
+            RG_HIP(hipHostMalloc(&caps->meta_pinned, hb.size(), hipHostMallocDefault));
+            caps->meta_pinned_cap = hb.size();
+        }
+        if (hb.size() > caps->meta_pinned_cap)
+            return fail(RG_ERR_ARGUMENT, "internal: graph table snapshot too small");
+        pinned = caps->meta_pinned;
+        caps->meta_bytes = hb.size();
+    } else {
+        RG_TRY(PinnedUpload::get(&pinned, hb.size()));
+        up.pending = true;
+    }
     std::memcpy(pinned, hb.data(), hb.size());
-    up.pending = true;
     RG_HIP(hipMemcpyAsync(meta.p, pinned, hb.size(), hipMemcpyHostToDevice, s));
     const Build* d_builds = (const Build*)(meta.p + o_builds);
     const GridDesc* d_grids = (const GridDesc*)(meta.p + o_grids);
@@ -1604,11 +1759,12 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
     RG_TRY(end.alloc(chunk_tot, s));
     RG_TRY(count.alloc(chunk_tot, s));
     RG_TRY(hist.alloc(4 * n_groups, s));
-    // anomalies, fixups, failed/overflow, long runs, restaged chunks; words
-    // 6-7: the triples beyond the staging capacity (64-bit)
-    RG_TRY(counters.alloc(8, s));
+    // anomalies, fixups, failed/overflow, long runs, restaged chunks, heavy
+    // groups; words 6-7: the triples beyond the staging capacity (64-bit);
+    // word 8: capacity bits of a capacity-mode build
+    RG_TRY(counters.alloc(12, s));
     RG_TRY(fill_zero(hist.p, 4 * n_groups, s));
-    RG_TRY(fill_zero(counters.p, 8, s));
+    RG_TRY(fill_zero(counters.p, 12, s));
     k_row_start<<<grid_for(row_tot * 64, 256), 256, 0, s>>>(
         d_builds, pair_of_row.p, row_tot, start.p, counters.p + 0);
     RG_HIP(hipGetLastError());
@@ -1644,23 +1800,36 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
         count.p, chunk_tot, cap, ov.p, (unsigned long long*)(counters.p + 6));
     RG_HIP(hipGetLastError());
     int64_t tot[2];
-    int32_t cnt[8];
-    RG_TRY(read_scalars(chunk_off.p + chunk_tot, &tot[0], sizeof(int64_t), s));
-    RG_TRY(read_scalars(bucket_off.p + 4 * n_groups, &tot[1], sizeof(int64_t), s));
-    RG_TRY(read_scalars(counters.p, cnt, sizeof(cnt), s));
+    int32_t cnt[8] = {};
     int64_t ov_tot;
-    std::memcpy(&ov_tot, cnt + 6, sizeof(ov_tot));
-    up.pending = false;  // the stream has passed the upload
+    int64_t T;
+    if (capmode) {
+        k_caps_check_triples<<<1, 1, 0, s>>>(chunk_off.p + chunk_tot, bucket_off.p + 4 * n_groups,
+                                             counters.p, caps->T_cap, caps->ov_cap);
+        RG_HIP(hipGetLastError());
+        T = caps->T_cap;
+        ov_tot = caps->ov_cap;
+    } else {
+        RG_TRY(read_scalars(chunk_off.p + chunk_tot, &tot[0], sizeof(int64_t), s));
+        RG_TRY(read_scalars(bucket_off.p + 4 * n_groups, &tot[1], sizeof(int64_t), s));
+        RG_TRY(read_scalars(counters.p, cnt, sizeof(cnt), s));
+        std::memcpy(&ov_tot, cnt + 6, sizeof(ov_tot));
+        up.pending = false;  // the stream has passed the upload
+        if (cnt[2] != 0)
+            return fail(RG_ERR_GEOMETRY,
+                        "the sweep did not terminate in " + std::to_string(cnt[2]) +
+                            " rows (exactly degenerate crossings)");
+        T = tot[0];
+        if (T != tot[1])
+            return fail(RG_ERR_HIP, "internal: triple count " + std::to_string(T) +
+                                        " != histogram total " +
+                                        std::to_string(tot[1]));
+        if (caps) {
+            caps->T_seen = T;
+            caps->ov_seen = ov_tot;
+        }
+    }
     tm.mark();  // 4
-    if (cnt[2] != 0)
-        return fail(RG_ERR_GEOMETRY,
-                    "the sweep did not terminate in " + std::to_string(cnt[2]) +
-                        " rows (exactly degenerate crossings)");
-    const int64_t T = tot[0];
-    if (T != tot[1])
-        return fail(RG_ERR_HIP, "internal: triple count " + std::to_string(T) +
-                                    " != histogram total " +
-                                    std::to_string(tot[1]));
     if (T >= ((int64_t)1 << 32))
         return fail(RG_ERR_ARGUMENT, "more than 2^32 triples in one build");
 
@@ -1677,17 +1846,17 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
     DevBuf<int64_t> ov_off;
     RG_TRY(ovbuf.alloc(ov_tot > 0 ? ov_tot : 1, s));
     RG_TRY(ov_off.alloc(chunk_tot + 1, s));
-    if (ov_tot > 0) {
+    if (ov_tot > 0 || capmode) {
         // the chunks' ranges of the overflow buffer
         RG_TRY(scan_i32_to_i64(ov.p, ov_off.p, chunk_tot, s));
         k_restage<<<grid_for(thr_tot, block), block, 0, s>>>(
             d_builds, pair_of_wave.p, thr_tot, start.p, count.p, ov_off.p, ovbuf.p, cap,
-            counters.p + 4);
+            ov_tot, counters.p + 4);
         RG_HIP(hipGetLastError());
     }
     k_scatter<<<grid_for(thr_tot, block), block, 0, s>>>(
         d_builds, pair_of_wave.p, thr_tot, n_groups, count.p, stage.p, cap, ovbuf.p,
-        ov_off.p, chunk_off.p, cursor, rec.p, rcell.p, (uint32_t)T, counters.p + 2);
+        ov_off.p, ov_tot, chunk_off.p, cursor, rec.p, rcell.p, (uint32_t)T, counters.p + 2);
     RG_HIP(hipGetLastError());
     tm.mark();  // 6
     k_cursor_check<<<grid_for(4 * n_groups, 256), 256, 0, s>>>(
@@ -1722,7 +1891,8 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
                                               rcell.p, red.p, cell_off.p, vol.p, wi_all.p,
                                               nnz_row.p, group_mode.p, long_list.p,
                                               counters.p + 3, heavy_list.p, counters.p + 5,
-                                              d_pcs, pair_of_ing.p, hist_out.p);
+                                              d_pcs, pair_of_ing.p, hist_out.p, T,
+                                              counters.p + 8);
     k_reduce_heavy<<<1024, RED_WAVES * 64, 0, s>>>(
         bucket_off.p, n_groups, group_off.p, rec.p, rcell.p, red.p, cell_off.p, vol.p,
         wi_all.p, nnz_row.p, group_mode.p, long_list.p, counters.p + 3, heavy_list.p,
@@ -1740,17 +1910,25 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
         d_pcs, d_builds, npairs, row_off.p, cell_tot, chunk_off.p, chunk_tot, bases.p);
     RG_HIP(hipGetLastError());
     std::vector<int64_t> hbases(2 * (npairs + 1));
-    RG_TRY(read_scalars(bases.p, hbases.data(), sizeof(int64_t) * hbases.size(), s));
-    int32_t cnt2[8];
-    RG_TRY(read_scalars(counters.p, cnt2, sizeof(cnt2), s));
-    if (cnt2[2] != 0)
-        return fail(RG_ERR_HIP, "internal: the emission did not match the "
-                                "counted buckets (" + std::to_string(cnt2[2]) +
-                                    " mismatches)");
-    const int64_t nnz = hbases[2 * npairs];
-    for (int64_t k = 0; k < npairs; k++)
-        if (hbases[2 * k + 2] - hbases[2 * k] >= INT32_MAX)
-            return fail(RG_ERR_ARGUMENT, "more than 2^31 - 1 weights in one element");
+    int32_t cnt2[8] = {};
+    int64_t nnz;
+    if (capmode) {
+        k_caps_check_weights<<<1, 1, 0, s>>>(row_off.p + cell_tot, counters.p, caps->nnz_cap);
+        RG_HIP(hipGetLastError());
+        nnz = caps->nnz_cap;
+    } else {
+        RG_TRY(read_scalars(bases.p, hbases.data(), sizeof(int64_t) * hbases.size(), s));
+        RG_TRY(read_scalars(counters.p, cnt2, sizeof(cnt2), s));
+        if (cnt2[2] != 0)
+            return fail(RG_ERR_HIP, "internal: the emission did not match the "
+                                    "counted buckets (" + std::to_string(cnt2[2]) +
+                                        " mismatches)");
+        nnz = hbases[2 * npairs];
+        if (caps) caps->nnz_seen = nnz;
+        for (int64_t k = 0; k < npairs; k++)
+            if (hbases[2 * k + 2] - hbases[2 * k] >= INT32_MAX)
+                return fail(RG_ERR_ARGUMENT, "more than 2^31 - 1 weights in one element");
+    }
 
     // ---- CSR by input, transpose -> CSR by output
     DevBuf<int32_t> rp_in, c_in, rp_out, c_out;
@@ -1767,14 +1945,40 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
     RG_TRY(scan_i32_to_i64(hist_out.p, off_out.p, out_tot, s));
     k_csr_groups<<<grid_for(n_groups * 64, 256), 256, 0, s>>>(
         d_pcs, pair_of_ing.p, n_groups, cell_off.p, row_off.p, group_mode.p,
-        red.p, rp_in.p, c_in.p, v_in.p, off_out.p, c_out.p, v_out.p);
+        red.p, rp_in.p, c_in.p, v_in.p, off_out.p, c_out.p, v_out.p, nnz);
     RG_HIP(hipGetLastError());
     const int64_t sort_groups = out_tot / 64;
     k_rows_sort<<<(unsigned)((sort_groups + SORT_WAVES - 1) / SORT_WAVES),
                   SORT_WAVES * 64, 0, s>>>(d_pcs, pair_of_outg.p, off_out.p, out_tot,
-                                           c_out.p, v_out.p, rp_out.p);
+                                           c_out.p, v_out.p, rp_out.p, nnz);
     RG_HIP(hipGetLastError());
     tm.mark();  // 9
+
+    if (capmode) {
+        // the graph's apply, then hand the device arrays to the graph (they
+        // live in its arena)
+        if (!gjobs.empty()) {
+            JobDesc* d_gjobs = (JobDesc*)(meta.p + o_gjobs);
+            k_graph_jobs<<<grid_for((int64_t)gjobs.size(), 64), 64, 0, s>>>(
+                d_gjobs, (const int32_t*)(meta.p + o_gpair), (int)gjobs.size(), d_pcs, bases.p,
+                rp_out.p, c_out.p, v_out.p);
+            RG_HIP(hipGetLastError());
+            RG_TRY(apply_jobs_device(d_gjobs, (const int64_t*)(meta.p + o_gprefix),
+                                     (int)gjobs.size(), gblocks, counters.p + 8, s));
+        }
+        caps->counters = counters.release();
+        caps->bases = bases.release();
+        caps->rp_in = rp_in.release();
+        caps->c_in = c_in.release();
+        caps->v_in = v_in.release();
+        caps->rp_out = rp_out.release();
+        caps->c_out = c_out.release();
+        caps->v_out = v_out.release();
+        caps->pcs = pcs;
+        caps->nchunks.resize(npairs);
+        for (int64_t k = 0; k < npairs; k++) caps->nchunks[k] = builds[k].nchunks;
+        return RG_OK;
+    }
 
     // ---- handles
     std::vector<rg_weights*> made;
@@ -1853,6 +2057,166 @@ int build_conservative_2d(const double* xi, const double* yi, int64_t nxi,
     d.wi = wi;
     if (!out) return fail(RG_ERR_ARGUMENT, "out is NULL");
     return build_batched(&d, 1, tuning, out, s);
+}
+
+// ---- hipGraph of a batched build + apply ------------------------------
+
+void weights_release(rg_weights* W);
+
+struct GraphImpl {
+    int device = 0;
+    int64_t npairs = 0;
+    std::vector<rg_pair_desc> pairs;
+    CaptureCaps caps;
+    void* arena = nullptr;
+    size_t arena_bytes = 0;
+    hipStream_t cs = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<GraphJob> jobs;
+    bool launched = false;  // the buffers hold a replay's results
+
+    ~GraphImpl()
+    {
+        (void)hipSetDevice(device);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (cs) {
+            (void)hipStreamSynchronize(cs);
+            (void)hipStreamDestroy(cs);
+        }
+        (void)hipDeviceSynchronize();  // replays on other streams
+        if (arena) (void)hipFree(arena);
+        if (caps.meta_pinned) (void)hipHostFree(caps.meta_pinned);
+    }
+};
+
+namespace {
+
+int64_t with_slack(int64_t n, double slack, int64_t floor_add)
+{
+    return (int64_t)((double)n * (1.0 + slack)) + floor_add;
+}
+
+// A copy of pair k's weights out of the graph's buffers.
+int graph_weights_copy(GraphImpl* G, int64_t k, rg_weights** out, hipStream_t s)
+{
+    if (!G->launched) return fail(RG_ERR_ARGUMENT, "the graph has not been launched");
+    int32_t flags = 0;
+    RG_TRY(read_scalars(G->caps.counters + 8, &flags, sizeof(flags), s));
+    if (flags)
+        return fail(RG_ERR_CAPACITY, "the graph's last replay exceeded its capacity (flags " +
+                                         std::to_string(flags) + ")");
+    std::vector<int64_t> hb(2 * (G->npairs + 1));
+    RG_TRY(read_scalars(G->caps.bases, hb.data(), sizeof(int64_t) * hb.size(), s));
+    if (hb[2 * G->npairs] > G->caps.nnz_cap || hb[2 * k] > hb[2 * k + 2])
+        return fail(RG_ERR_HIP, "internal: graph weight offsets out of range");
+    const PairCells& P = G->caps.pcs[k];
+    const rg_pair_desc& d = G->pairs[k];
+    const int64_t eb = hb[2 * k], nnz = hb[2 * k + 2] - eb;
+    DevBuf<int32_t> rp_in, c_in, rp_out, c_out;
+    DevBuf<double> v_in, v_out;
+    RG_TRY(rp_in.alloc(P.n_in + 1, s));
+    RG_TRY(rp_out.alloc(P.n_out + 1, s));
+    RG_TRY(c_in.alloc(nnz > 0 ? nnz : 1, s));
+    RG_TRY(c_out.alloc(nnz > 0 ? nnz : 1, s));
+    RG_TRY(v_in.alloc(nnz > 0 ? nnz : 1, s));
+    RG_TRY(v_out.alloc(nnz > 0 ? nnz : 1, s));
+    auto cp = [&](void* dst, const void* src, size_t bytes) -> int {
+        if (bytes) RG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s));
+        return RG_OK;
+    };
+    RG_TRY(cp(rp_in.p, G->caps.rp_in + P.cell_gbase + k, 4 * (P.n_in + 1)));
+    RG_TRY(cp(rp_out.p, G->caps.rp_out + P.out_gbase + k, 4 * (P.n_out + 1)));
+    RG_TRY(cp(c_in.p, G->caps.c_in + eb, 4 * nnz));
+    RG_TRY(cp(c_out.p, G->caps.c_out + eb, 4 * nnz));
+    RG_TRY(cp(v_in.p, G->caps.v_in + eb, 8 * nnz));
+    RG_TRY(cp(v_out.p, G->caps.v_out + eb, 8 * nnz));
+    RG_HIP(hipStreamSynchronize(s));
+    rg_weights* W = new rg_weights();
+    W->device = G->device;
+    W->n_in = P.n_in;
+    W->n_out = P.n_out;
+    W->nnz = nnz;
+    W->shape_in[0] = d.nxi - 1;
+    W->shape_in[1] = d.nyi - 1;
+    W->shape_out[0] = d.nxo - 1;
+    W->shape_out[1] = d.nyo - 1;
+    auto arena = std::make_shared<rg_arena>();
+    W->rowptr_in = rp_in.release();
+    W->col_in = c_in.release();
+    W->val_in = v_in.release();
+    W->rowptr_out = rp_out.release();
+    W->col_out = c_out.release();
+    W->val_out = v_out.release();
+    arena->bufs = {W->rowptr_in, W->col_in, W->val_in, W->rowptr_out, W->col_out, W->val_out};
+    W->arena = arena;
+    W->stats.n_triples = hb[2 * k + 3] - hb[2 * k + 1];
+    W->stats.n_subsegments = W->stats.n_triples / 2;
+    W->stats.n_chunks = G->caps.nchunks[k];
+    *out = W;
+    return RG_OK;
+}
+
+}  // namespace
+
+int graph_create(const rg_pair_desc* pairs, int64_t npairs, const rg_tuning* tuning,
+                 const GraphJob* jobs, double slack, GraphImpl** out, hipStream_t s)
+{
+    *out = nullptr;
+    if (!(slack >= -0.99 && slack <= 100)) return fail(RG_ERR_ARGUMENT, "slack out of range");
+    if (npairs <= 0 || !pairs) return fail(RG_ERR_ARGUMENT, "no pairs");
+    auto G = std::make_unique<GraphImpl>();
+    RG_HIP(hipGetDevice(&G->device));
+    G->npairs = npairs;
+    G->pairs.assign(pairs, pairs + npairs);
+    if (jobs) G->jobs.assign(jobs, jobs + npairs);
+    rg_tuning t = {};
+    if (tuning) t = *tuning;
+    t.flags &= ~RG_TUNE_TIMINGS;
+    // 1. an ordinary build measures the data-dependent sizes
+    {
+        std::vector<rg_weights*> hs(npairs, nullptr);
+        CaptureCaps sizing;
+        const int st = build_batched(pairs, npairs, &t, hs.data(), s, &sizing);
+        for (rg_weights* w : hs) weights_release(w);
+        if (st != RG_OK) return st;
+        G->caps.T_cap = with_slack(sizing.T_seen, slack, 4096);
+        G->caps.ov_cap = with_slack(sizing.ov_seen, slack, 4096);
+        G->caps.nnz_cap = with_slack(sizing.nnz_seen, slack, 4096);
+        if (G->caps.T_cap >= ((int64_t)1 << 32))
+            return fail(RG_ERR_ARGUMENT, "more than 2^32 triples in one graph");
+    }
+    G->caps.capacity = true;
+    G->caps.jobs = jobs ? G->jobs.data() : nullptr;
+    // 2. a capacity-mode run, not captured, adds up the workspace it needs
+    //    (and makes the table snapshot)
+    {
+        arena_measure_begin();
+        const int st = build_batched(pairs, npairs, &t, nullptr, s, &G->caps);
+        G->arena_bytes = arena_measure_end() + (1 << 20);
+        if (st != RG_OK) return st;
+        RG_HIP(hipStreamSynchronize(s));
+        for (void* p : {(void*)G->caps.counters, (void*)G->caps.bases, (void*)G->caps.rp_in,
+                        (void*)G->caps.c_in, (void*)G->caps.v_in, (void*)G->caps.rp_out,
+                        (void*)G->caps.c_out, (void*)G->caps.v_out})
+            dev_free(p);
+    }
+    // 3. capture the same run, its buffers bumped out of one workspace
+    RG_HIP(hipMalloc(&G->arena, G->arena_bytes));
+    RG_HIP(hipStreamCreateWithFlags(&G->cs, hipStreamNonBlocking));
+    RG_HIP(hipStreamBeginCapture(G->cs, hipStreamCaptureModeThreadLocal));
+    arena_begin(G->arena, G->arena_bytes);
+    const int st = build_batched(pairs, npairs, &t, nullptr, G->cs, &G->caps);
+    arena_end();
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(G->cs, &graph);
+    G->graph = graph;
+    if (st != RG_OK) return st;
+    RG_HIP(e);
+    RG_HIP(hipGraphInstantiate(&G->exec, G->graph, nullptr, nullptr, 0));
+    *out = G.release();
+    return RG_OK;
 }
 
 }  // namespace rg
@@ -2200,6 +2564,61 @@ int rg_weights_set_shape(rg_weights_t w, int64_t ncx_in, int64_t ncy_in,
 int rg_weights_free(rg_weights_t w)
 {
     weights_release(w);
+    return RG_OK;
+}
+
+int rg_graph_create(const rg_pair_desc* pairs, int64_t npairs, const rg_tuning* tuning,
+                    const rg_graph_job* jobs, double slack, rg_graph_t* out, void* stream)
+{
+    if (!out) return fail(RG_ERR_ARGUMENT, "out is NULL");
+    static_assert(sizeof(rg_graph_job) == sizeof(GraphJob), "job layout");
+    GraphImpl* G = nullptr;
+    RG_TRY(graph_create(pairs, npairs, tuning, (const GraphJob*)jobs, slack, &G,
+                        (hipStream_t)stream));
+    *out = (rg_graph_t)G;
+    return RG_OK;
+}
+
+int rg_graph_launch(rg_graph_t g, void* stream)
+{
+    if (!g) return fail(RG_ERR_ARGUMENT, "graph is NULL");
+    GraphImpl* G = (GraphImpl*)g;
+    RG_HIP(hipGraphLaunch(G->exec, (hipStream_t)stream));
+    G->launched = true;
+    return RG_OK;
+}
+
+int rg_graph_status(rg_graph_t g, int32_t* flags, void* stream)
+{
+    if (!g || !flags) return fail(RG_ERR_ARGUMENT, "graph or flags is NULL");
+    GraphImpl* G = (GraphImpl*)g;
+    if (!G->launched) return fail(RG_ERR_ARGUMENT, "the graph has not been launched");
+    return read_scalars(G->caps.counters + 8, flags, sizeof(int32_t), (hipStream_t)stream);
+}
+
+int rg_graph_weights(rg_graph_t g, int64_t k, rg_weights_t* out, void* stream)
+{
+    if (!g || !out) return fail(RG_ERR_ARGUMENT, "graph or out is NULL");
+    GraphImpl* G = (GraphImpl*)g;
+    if (k < 0 || k >= G->npairs) return fail(RG_ERR_ARGUMENT, "pair index out of range");
+    return graph_weights_copy(G, k, out, (hipStream_t)stream);
+}
+
+int rg_graph_capacity(rg_graph_t g, int64_t* triples, int64_t* overflow, int64_t* weights,
+                      int64_t* workspace_bytes)
+{
+    if (!g) return fail(RG_ERR_ARGUMENT, "graph is NULL");
+    GraphImpl* G = (GraphImpl*)g;
+    if (triples) *triples = G->caps.T_cap;
+    if (overflow) *overflow = G->caps.ov_cap;
+    if (weights) *weights = G->caps.nnz_cap;
+    if (workspace_bytes) *workspace_bytes = (int64_t)G->arena_bytes;
+    return RG_OK;
+}
+
+int rg_graph_destroy(rg_graph_t g)
+{
+    delete (GraphImpl*)g;
     return RG_OK;
 }
 

@@ -44,6 +44,11 @@ int fail(int code, const std::string& msg);
 int dev_alloc(void** p, size_t bytes, hipStream_t s);
 void dev_free(void* p);
 void dev_trim();
+// Arena mode of the allocator for the calling host thread (see rg_runtime.hip).
+void arena_begin(void* base, size_t cap);
+size_t arena_end();
+void arena_measure_begin();
+size_t arena_measure_end();
 
 template <class T>
 struct DevBuf {
@@ -117,6 +122,28 @@ __device__ inline int seg_of(const int64_t* __restrict__ base, int n, int64_t x)
     }
     return lo;
 }
+}  // namespace rg
+
+namespace rg {
+// One job of a batched apply (rg_apply.hip): CSR rows of one weight set
+// applied to `batch` value arrays; cut into blocks of rows x images.
+struct JobDesc {
+    const int32_t* rowptr;
+    const int32_t* col;
+    const void* w;
+    const void* x;
+    void* y;
+    int64_t nrows, ncols, batch;
+    int64_t block_base;  // first block of this job
+    int64_t nrt;         // row tiles of the job
+};
+constexpr int JOB_DB = 8;  // images per block
+// Launches the FP64 batched apply over a device job table (block prefix
+// `prefix`, `nblocks` blocks in all); every block returns at once when
+// *abort_flag is nonzero (abort_flag may be null).
+int apply_jobs_device(const JobDesc* jobs, const int64_t* prefix, int njobs, int64_t nblocks,
+                      const int32_t* abort_flag, hipStream_t s);
+int64_t apply_block_rows();  // output rows per block of the batched apply
 }  // namespace rg
 
 struct rg_arena {

@@ -83,11 +83,64 @@ size_t round_size(size_t bytes)
 
 }  // namespace
 
+// Arena mode (per host thread): allocations bump through one caller-owned
+// device range and frees are no-ops, so a build can run under stream capture
+// with no allocation call and its buffers outlive it.  Measure mode adds up
+// what a run requests (in arena granules) while allocating normally.
+namespace {
+constexpr size_t ARENA_ALIGN = 256;
+struct ArenaCtx {
+    char* base = nullptr;
+    size_t cap = 0, used = 0;
+    bool measure = false;
+    size_t measured = 0;
+};
+thread_local ArenaCtx g_arena;
+}  // namespace
+
+void arena_begin(void* base, size_t cap)
+{
+    g_arena.base = (char*)base;
+    g_arena.cap = cap;
+    g_arena.used = 0;
+}
+
+size_t arena_end()
+{
+    const size_t used = g_arena.used;
+    g_arena.base = nullptr;
+    g_arena.cap = g_arena.used = 0;
+    return used;
+}
+
+void arena_measure_begin()
+{
+    g_arena.measure = true;
+    g_arena.measured = 0;
+}
+
+size_t arena_measure_end()
+{
+    g_arena.measure = false;
+    return g_arena.measured;
+}
+
 int dev_alloc(void** p, size_t bytes, hipStream_t s)
 {
     *p = nullptr;
     if (injected_failure("alloc"))
         return fail(RG_ERR_INJECTED, "injected allocation failure");
+    if (g_arena.base) {
+        const size_t sz = (bytes + ARENA_ALIGN - 1) / ARENA_ALIGN * ARENA_ALIGN;
+        if (g_arena.used + sz > g_arena.cap)
+            return fail(RG_ERR_MEMORY, "graph workspace exhausted (" +
+                                           std::to_string(g_arena.used + sz) + " > " +
+                                           std::to_string(g_arena.cap) + " bytes)");
+        *p = g_arena.base + g_arena.used;
+        g_arena.used += sz;
+        return RG_OK;
+    }
+    if (g_arena.measure) g_arena.measured += (bytes + ARENA_ALIGN - 1) / ARENA_ALIGN * ARENA_ALIGN;
     int dev = 0;
     RG_HIP(hipGetDevice(&dev));
     const BlockTag tag{dev, s};
@@ -140,6 +193,8 @@ int dev_alloc(void** p, size_t bytes, hipStream_t s)
 void dev_free(void* p)
 {
     if (!p) return;
+    if (g_arena.base && (char*)p >= g_arena.base && (char*)p < g_arena.base + g_arena.cap)
+        return;  // arena memory lives as long as its owner
     Pool& P = pool();
     std::lock_guard<std::mutex> lk(P.mu);
     auto it = P.live.find(p);
@@ -258,6 +313,8 @@ __global__ void k_set_total(int64_t* out, int64_t n, const int64_t* tile_off,
     out[n] = tile_off[ntiles];
 }
 
+__global__ void k_zero_total(int64_t* out) { out[0] = 0; }
+
 __global__ void k_copy_total(int64_t* out, int64_t n, const int64_t* tile_sums)
 {
     out[n] = tile_sums[0];
@@ -266,8 +323,9 @@ __global__ void k_copy_total(int64_t* out, int64_t n, const int64_t* tile_sums)
 template <class T>
 int scan_impl(const T* in, int64_t* out, int64_t n, hipStream_t s)
 {
-    if (n == 0) {
-        RG_HIP(hipMemsetAsync(out, 0, sizeof(int64_t), s));
+    if (n == 0) {  // a kernel, not a memset node, when captured in a graph
+        k_zero_total<<<1, 1, 0, s>>>(out);
+        RG_HIP(hipGetLastError());
         return RG_OK;
     }
     int64_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;

@@ -535,7 +535,7 @@ RG_HD double pairwise_sum(const double* a, int64_t n, int64_t stride = 1)
 
 // numpy's split of a pairwise-sum node of n > 128 elements: the left child
 // takes n/2 rounded down to a multiple of 8.
-RG_HD inline int64_t pairwise_split(int64_t n)
+RG_HD int64_t pairwise_split(int64_t n)
 {
     const int64_t n2 = n / 2;
     return n2 - n2 % 8;
@@ -748,7 +748,7 @@ RG_HD int64_t bucket_reduce(uint64_t* keys, double* vals, int64_t n,
 // Bitonic sorting network over N keys in registers (N a power of two, every
 // index static once unrolled), values following.
 template <int N, class K>
-RG_HD inline void sort_network(K* k, double* v)
+RG_HD void sort_network(K* k, double* v)
 {
 #pragma unroll
     for (int size = 2; size <= N; size <<= 1) {
