@@ -404,6 +404,26 @@ def bench_batched(args) -> dict:
         h.free()
     same = bool(np.array_equal(dYp.to_host(), dYb.to_host()))
     ratio = ms_plain / ms_batched
+    # the same build + apply captured once in a hipGraph, replayed
+    from regridding_amd.graph import BuildApplyGraph
+
+    dYg = DeviceArray.empty((int(y_off[-1]),), np.float64)
+    t2 = time.perf_counter()
+    graph = BuildApplyGraph(
+        packed, slack=0.25, edges_per_lane=tuned["edges_per_lane"], block=tuned["block"],
+        device_jobs=[(dX.ptr + 8 * int(x_off[k]), dYg.ptr + 8 * int(y_off[k]), nb[k])
+                     for k in range(n_pairs)])
+    s_graph_create = time.perf_counter() - t2
+    ms_graph = timed(graph.launch, 5)
+    graph.check()
+    same_graph = bool(np.array_equal(dYg.to_host(), dYb.to_host()))
+    graph_caps = graph.capacity()
+    graph.close()
+    del dYg
+    log(f"graph: replay {ms_graph:.2f} ms ({n_pairs / ms_graph * 1e3:.0f} pairs/s, "
+        f"x{ms_plain / ms_graph:.2f} over plain), created in {s_graph_create:.2f} s, "
+        f"workspace {graph_caps['workspace_bytes'] / 2**30:.2f} GiB, outputs identical: "
+        f"{same_graph}")
     log(f"batched: plain {ms_plain:.2f} ms ({n_pairs / ms_plain * 1e3:.0f} pairs/s), batched "
         f"{ms_batched:.2f} ms ({n_pairs / ms_batched * 1e3:.0f} pairs/s), x{ratio:.2f}; "
         f"outputs identical: {same}; batch phases "
@@ -423,6 +443,12 @@ def bench_batched(args) -> dict:
         "batched_pairs_per_s": round(n_pairs / ms_batched * 1e3, 1),
         "batched_vs_plain": round(ratio, 3),
         "outputs_bit_identical": same,
+        "graph_ms": round(ms_graph, 3),
+        "graph_pairs_per_s": round(n_pairs / ms_graph * 1e3, 1),
+        "graph_vs_plain": round(ms_plain / ms_graph, 3),
+        "graph_outputs_bit_identical": same_graph,
+        "graph_create_s": round(s_graph_create, 3),
+        "graph_workspace_bytes": int(graph_caps["workspace_bytes"]),
         "batch_build_phases_ms": {k[3:]: round(v, 3) for k, v in st.items() if k.startswith("ms_")},
         "batch_autotune": tuned,
     }

@@ -31,8 +31,9 @@ class BuildApplyGraph:
     counts, for replays whose coordinates produce more of them.
     """
 
-    def __init__(self, pairs: PackedPairs, values, slack: float = 0.25,
-                 edges_per_lane: int = 0, block: int = 0, stream: int = 0):
+    def __init__(self, pairs: PackedPairs, values=None, slack: float = 0.25,
+                 edges_per_lane: int = 0, block: int = 0, stream: int = 0,
+                 device_jobs=None):
         self.pairs = pairs
         L = lib()
         n = pairs.n
@@ -41,6 +42,18 @@ class BuildApplyGraph:
             d = pairs.desc[k]
             self.n_in.append((d.nxi - 1) * (d.nyi - 1))
             self.n_out.append((d.nxo - 1) * (d.nyo - 1))
+        t = rg_tuning(int(edges_per_lane), int(block), 0, 0)
+        jobs = (rg_graph_job * n)()
+        self.stream = stream
+        if device_jobs is not None:
+            # (input pointer, output pointer, batch) per pair: device arrays
+            # the caller owns and keeps alive while the graph exists
+            self.x = self.y = None
+            for k, (xp, yp, b) in enumerate(device_jobs):
+                jobs[k].values_input, jobs[k].values_output, jobs[k].batch = xp, yp, int(b)
+                self.batch.append(int(b))
+            self._create(pairs, t, jobs, slack, stream)
+            return
         x_parts, self._x_off, self._y_off = [], [], []
         xo = yo = 0
         for k, v in enumerate(values):
@@ -56,20 +69,23 @@ class BuildApplyGraph:
             yo += v.shape[0] * self.n_out[k]
         self.x = DeviceArray.from_host(np.concatenate(x_parts) if xo else np.zeros(1))
         self.y = DeviceArray.empty((max(yo, 1),), np.float64)
-        jobs = (rg_graph_job * n)()
         for k in range(n):
             jobs[k].values_input = self.x.ptr + 8 * self._x_off[k]
             jobs[k].values_output = self.y.ptr + 8 * self._y_off[k]
             jobs[k].batch = self.batch[k]
-        t = rg_tuning(int(edges_per_lane), int(block), 0, 0)
+        self._create(pairs, t, jobs, slack, stream)
+
+    def _create(self, pairs, t, jobs, slack, stream):
         h = ctypes.c_void_p()
-        check(L.rg_graph_create(pairs.desc, n, ctypes.byref(t), jobs, float(slack),
-                                ctypes.byref(h), stream))
+        self.ptr = None
+        check(lib().rg_graph_create(pairs.desc, pairs.n, ctypes.byref(t), jobs, float(slack),
+                                    ctypes.byref(h), stream))
         self.ptr = h.value
-        self.stream = stream
 
     def set_values(self, values) -> None:
         """New values of the same shapes for the next replays."""
+        if self.x is None:
+            raise ValueError("the graph applies to the caller's device arrays")
         parts = [np.asarray(v, dtype=np.float64).ravel() for v in values]
         flat = np.concatenate(parts) if parts else np.zeros(1)
         self.x.copy_from_host(flat if flat.size else np.zeros(1))
@@ -92,6 +108,8 @@ class BuildApplyGraph:
 
     def outputs(self) -> list[np.ndarray]:
         """Per pair the (batch, n_out) results of the last replay."""
+        if self.y is None:
+            raise ValueError("the graph applies to the caller's device arrays")
         y = self.y.to_host(self.stream)
         return [y[o: o + b * m].reshape(b, m)
                 for o, b, m in zip(self._y_off, self.batch, self.n_out)]
