@@ -6,11 +6,13 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <unordered_map>
 #include <set>
 #include <string>
 #include <vector>
 
 #include "rg_internal.hpp"
+#include "rg_layout.hpp"
 
 namespace rg {
 
@@ -83,46 +85,60 @@ size_t round_size(size_t bytes)
 
 }  // namespace
 
-// Arena mode (per host thread): allocations bump through one caller-owned
-// device range and frees are no-ops, so a build can run under stream capture
-// with no allocation call and its buffers outlive it.  Measure mode adds up
-// what a run requests (in arena granules) while allocating normally.
+// Arena mode (per host thread): allocations are carved out of one
+// caller-owned device range, so a build can run under stream capture with no
+// allocation call, and what it does not free outlives it.  A freed range is
+// reused by later allocations (best fit, neighbours merged): on one stream,
+// and so in the chain a capture of that stream records, the later work runs
+// after the earlier.  Measure mode lays the requests of an ordinary run out
+// the same way, beside the real allocations, and reports the highest offset
+// reached: a run that makes the same requests in the same order fits a
+// range of that size.
 namespace {
 constexpr size_t ARENA_ALIGN = 256;
+
 struct ArenaCtx {
     char* base = nullptr;
-    size_t cap = 0, used = 0;
+    size_t cap = 0;
     bool measure = false;
-    size_t measured = 0;
+    OffsetLayout layout;
+    std::unordered_map<void*, size_t> measured_at;  // real block -> its offset
 };
 thread_local ArenaCtx g_arena;
+
+size_t arena_round(size_t bytes) { return (bytes + ARENA_ALIGN - 1) / ARENA_ALIGN * ARENA_ALIGN; }
 }  // namespace
 
 void arena_begin(void* base, size_t cap)
 {
     g_arena.base = (char*)base;
     g_arena.cap = cap;
-    g_arena.used = 0;
+    g_arena.layout.reset();
 }
 
 size_t arena_end()
 {
-    const size_t used = g_arena.used;
+    const size_t peak = g_arena.layout.peak;
     g_arena.base = nullptr;
-    g_arena.cap = g_arena.used = 0;
-    return used;
+    g_arena.cap = 0;
+    g_arena.layout.reset();
+    return peak;
 }
 
 void arena_measure_begin()
 {
     g_arena.measure = true;
-    g_arena.measured = 0;
+    g_arena.layout.reset();
+    g_arena.measured_at.clear();
 }
 
 size_t arena_measure_end()
 {
     g_arena.measure = false;
-    return g_arena.measured;
+    const size_t peak = g_arena.layout.peak;
+    g_arena.layout.reset();
+    g_arena.measured_at.clear();
+    return peak;
 }
 
 int dev_alloc(void** p, size_t bytes, hipStream_t s)
@@ -131,16 +147,17 @@ int dev_alloc(void** p, size_t bytes, hipStream_t s)
     if (injected_failure("alloc"))
         return fail(RG_ERR_INJECTED, "injected allocation failure");
     if (g_arena.base) {
-        const size_t sz = (bytes + ARENA_ALIGN - 1) / ARENA_ALIGN * ARENA_ALIGN;
-        if (g_arena.used + sz > g_arena.cap)
+        const size_t sz = arena_round(bytes);
+        const size_t off = g_arena.layout.take(sz);
+        if (off + sz > g_arena.cap) {
+            g_arena.layout.give(off);
             return fail(RG_ERR_MEMORY, "graph workspace exhausted (" +
-                                           std::to_string(g_arena.used + sz) + " > " +
+                                           std::to_string(off + sz) + " > " +
                                            std::to_string(g_arena.cap) + " bytes)");
-        *p = g_arena.base + g_arena.used;
-        g_arena.used += sz;
+        }
+        *p = g_arena.base + off;
         return RG_OK;
     }
-    if (g_arena.measure) g_arena.measured += (bytes + ARENA_ALIGN - 1) / ARENA_ALIGN * ARENA_ALIGN;
     int dev = 0;
     RG_HIP(hipGetDevice(&dev));
     const BlockTag tag{dev, s};
@@ -156,6 +173,7 @@ int dev_alloc(void** p, size_t bytes, hipStream_t s)
             *p = it->second;
             P.live[*p] = LiveBlock{tag, it->first};
             fb.erase(it);
+            if (g_arena.measure) g_arena.measured_at[*p] = g_arena.layout.take(arena_round(bytes));
             return RG_OK;
         }
     }
@@ -187,14 +205,24 @@ int dev_alloc(void** p, size_t bytes, hipStream_t s)
     std::lock_guard<std::mutex> lk(P.mu);
     P.live[q] = LiveBlock{tag, sz};
     *p = q;
+    if (g_arena.measure) g_arena.measured_at[q] = g_arena.layout.take(arena_round(bytes));
     return RG_OK;
 }
 
 void dev_free(void* p)
 {
     if (!p) return;
-    if (g_arena.base && (char*)p >= g_arena.base && (char*)p < g_arena.base + g_arena.cap)
-        return;  // arena memory lives as long as its owner
+    if (g_arena.base && (char*)p >= g_arena.base && (char*)p < g_arena.base + g_arena.cap) {
+        g_arena.layout.give((size_t)((char*)p - g_arena.base));
+        return;
+    }
+    if (g_arena.measure) {
+        auto m = g_arena.measured_at.find(p);
+        if (m != g_arena.measured_at.end()) {
+            g_arena.layout.give(m->second);
+            g_arena.measured_at.erase(m);
+        }
+    }
     Pool& P = pool();
     std::lock_guard<std::mutex> lk(P.mu);
     auto it = P.live.find(p);

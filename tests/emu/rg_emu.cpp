@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <vector>
 
+#include "../../regridding_amd/csrc/rg_layout.hpp"
 #include "../../regridding_amd/csrc/rg_sweep.hpp"
 
 using namespace rg;
@@ -433,6 +434,24 @@ int64_t rge_index_secant(const double* x, const double* y, int64_t nx,
                          int64_t ny, double px, double py)
 {
     return index_secant(x, y, nx, ny, px, py);
+}
+
+// The device-range layout of the graph workspace (rg_layout.hpp) over a
+// trace of requests: ops[q] > 0 allocates that many bytes (its offset goes
+// to off[q]), ops[q] < 0 frees the allocation made by op -ops[q] - 1.
+// Returns the peak.
+int64_t rge_layout_trace(const int64_t* ops, int64_t n, int64_t* off)
+{
+    OffsetLayout L;
+    for (int64_t q = 0; q < n; q++) {
+        if (ops[q] > 0) {
+            off[q] = (int64_t)L.take((size_t)ops[q]);
+        } else {
+            L.give((size_t)off[-ops[q] - 1]);
+            off[q] = -1;
+        }
+    }
+    return (int64_t)L.peak;
 }
 
 }  // extern "C"
