@@ -944,11 +944,17 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
 }
 
 // Groups beyond the LDS share, one wavefront each (grid-stride over the
-// list): the group is copied cell-major into `red` (global), then goes
-// through LDS in runs of consecutive cells, each cell reduced in place at
-// its own offset (the group is marked unpacked); a single cell beyond the
-// share is left to k_bucket_long.  Cells of up to 2 RED_NET records sort in
-// registers.
+// list).  The cells' counts give their cell-major offsets in `red` (the
+// group is marked unpacked); the cells are then taken in runs of
+// consecutive cells that fit the LDS share.  For each run the wave reads
+// the group's cell bytes and places the records of the run's cells in LDS,
+// cell-major; every record finds its place in its cell's key order by
+// counting the smaller keys of its cell (the keys are unique: they hold the
+// emission ordinal), and each lane reduces one cell at its own offset.  A
+// single cell beyond the share is copied cell-major into `red` and left to
+// k_bucket_long.
+constexpr int HEAVY_CAP = RED_CAP;  // LDS entries per wavefront
+
 __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_heavy(
     const int64_t* __restrict__ bucket_off, int64_t n_groups,
     const int64_t* __restrict__ group_off, const KV* __restrict__ rec,
@@ -960,13 +966,16 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_heavy(
     const int32_t* __restrict__ n_heavy, const PairCells* __restrict__ pcs,
     const int32_t* __restrict__ pair_of_group, int32_t* __restrict__ hist_out_all)
 {
-    __shared__ uint64_t sk[RED_WAVES][RED_CAP];
-    __shared__ double sv[RED_WAVES][RED_CAP];
+    __shared__ uint64_t sk[RED_WAVES][HEAVY_CAP];
+    __shared__ double sv[RED_WAVES][HEAVY_CAP];
+    __shared__ uint8_t sc[RED_WAVES][HEAVY_CAP];
     __shared__ int32_t scnt[RED_WAVES][64];
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
     int32_t* C = scnt[wid];
     uint64_t* K = sk[wid];
     double* V = sv[wid];
+    uint8_t* SC = sc[wid];
+    constexpr int U = HEAVY_CAP / 64;
     const int32_t nh = *n_heavy;
     for (int64_t h = (int64_t)blockIdx.x * RED_WAVES + wid; h < nh;
          h += (int64_t)gridDim.x * RED_WAVES) {  // uniform per wavefront
@@ -975,6 +984,8 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_heavy(
         const int64_t base = group_off[grp];
         const int64_t total = group_off[grp + 1] - base;
         const GroupRecs G(bucket_off, n_groups, grp);
+        int32_t* hist_out =
+            hist_out_all + pcs[__builtin_amdgcn_readfirstlane(pair_of_group[grp])].out_gbase;
         C[lane] = 0;
         wave_sync_lds();
         for (int64_t q0 = lane; q0 < total; q0 += 256) {
@@ -992,54 +1003,84 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_heavy(
         cell_off[i] = base + s;
         const double vol = vol_in[i];
         wave_sync_lds();
-        C[lane] = s;  // placement cursors
-        wave_sync_lds();
-        // cell-major copy of the group in `red`, read back by the wave's lanes
-        for (int64_t q0 = lane; q0 < total; q0 += 128) {
-            // two records in flight per lane
-            const int64_t qa = q0, qb = q0 + 64;
-            const int64_t sa = G.src(qa), sb2 = qb < total ? G.src(qb) : sa;
-            const int32_t ca = rcell[sa];
-            const KV ra = rec[sa];
-            const int32_t cb = rcell[sb2];
-            const KV rb = rec[sb2];
-            red[base + atomicAdd(&C[ca], 1)] = ra;
-            if (qb < total) red[base + atomicAdd(&C[cb], 1)] = rb;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         int c = 0;  // first cell of the next run
         while (c < 64) {
             const int32_t sb = __shfl(s, c, 64);  // wave-uniform
-            const bool in = lane >= c && s + n - sb <= RED_CAP;
+            const bool in = lane >= c && s + n - sb <= HEAVY_CAP;
             const uint64_t mask = __ballot(in);
             if (mask == 0) {
-                // a cell beyond the LDS share: finished by k_bucket_long
-                // (flagged in the long list)
+                // a cell beyond the LDS share: its records go cell-major to
+                // `red`, finished by k_bucket_long (flagged in the long list)
+                if (lane == 0) C[0] = 0;
+                wave_sync_lds();
+                for (int64_t q = lane; q < total; q += 64) {
+                    const int64_t src = G.src(q);
+                    if (rcell[src] == c) red[base + sb + atomicAdd(&C[0], 1)] = rec[src];
+                }
                 if (lane == c) long_list[atomicAdd(n_long, 1)] = (int32_t)i | INT32_MIN;
+                wave_sync_lds();
                 c++;
                 continue;
             }
             const int d = c + __popcll(mask);  // cells [c, d) fit
-            const int32_t se = __shfl(s + n, d - 1, 64);
-            for (int32_t q = lane; q < se - sb; q += 64) {
-                const KV r = red[base + sb + q];
-                K[q] = r.key;
-                V[q] = r.val;
+            const int32_t cnt = __shfl(s + n, d - 1, 64) - sb;
+            C[lane] = s - sb;  // placement cursors of the run's cells
+            wave_sync_lds();
+            // the run's records, cell-major in LDS
+            for (int64_t q0 = 0; q0 < total; q0 += 128) {  // uniform trip count
+                const int64_t qa = q0 + lane, qb = qa + 64;
+                const int64_t sa = qa < total ? G.src(qa) : -1;
+                const int64_t sb2 = qb < total ? G.src(qb) : -1;
+                const int32_t ca = sa >= 0 ? rcell[sa] : -1;
+                const int32_t cb = sb2 >= 0 ? rcell[sb2] : -1;
+                if (ca >= c && ca < d) {
+                    const KV r = rec[sa];
+                    const int32_t p = atomicAdd(&C[ca], 1);
+                    K[p] = r.key;
+                    V[p] = r.val;
+                    SC[p] = (uint8_t)ca;
+                }
+                if (cb >= c && cb < d) {
+                    const KV r = rec[sb2];
+                    const int32_t p = atomicAdd(&C[cb], 1);
+                    K[p] = r.key;
+                    V[p] = r.val;
+                    SC[p] = (uint8_t)cb;
+                }
             }
+            wave_sync_lds();
+            // each record's place in its cell: the number of smaller keys
+            uint64_t ke[U];
+            double ve[U];
+            int32_t pe[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int32_t e = lane + 64 * u;
+                const int32_t ce = e < cnt ? SC[e] : lane;
+                const int32_t cs = __shfl(s, ce, 64) - sb;
+                const int32_t cn = __shfl(n, ce, 64);
+                const double cv = __shfl(vol, ce, 64);
+                pe[u] = -1;
+                if (e < cnt) {
+                    const uint64_t k = K[e];
+                    int32_t r = 0;
+                    for (int32_t j = cs; j < cs + cn; j++) r += K[j] < k ? 1 : 0;
+                    ke[u] = k;
+                    ve[u] = weight_of(V[e], cv, wi, grp * 64 + ce);
+                    pe[u] = cs + r;
+                }
+            }
+            wave_sync_lds();
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (pe[u] >= 0) {
+                    K[pe[u]] = ke[u];
+                    V[pe[u]] = ve[u];
+                }
             wave_sync_lds();
             if (in) {
                 uint64_t* Kc = K + (s - sb);
                 double* Vc = V + (s - sb);
-                if (n <= RED_NET) {
-                    sort_cell_in_registers<RED_NET>(Kc, Vc, n, vol, wi, i);
-                } else if (n <= 2 * RED_NET) {
-                    sort_cell_in_registers<2 * RED_NET>(Kc, Vc, n, vol, wi, i);
-                } else {
-                    for (int32_t q = 0; q < n; q++) Vc[q] = weight_of(Vc[q], vol, wi, i);
-                    sort_pairs(Kc, Vc, (int64_t)n);
-                }
                 bool lrc = false;
                 int64_t m = bucket_reduce_sorted(Kc, Vc, n, &lrc);
                 if (lrc) {
@@ -1051,9 +1092,6 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_heavy(
                         red[base + s + t] = r;
                     }
                 } else {
-                    int32_t* hist_out =
-                        hist_out_all +
-                        pcs[__builtin_amdgcn_readfirstlane(pair_of_group[grp])].out_gbase;
                     for (int64_t t = 0; t < m; t++)
                         put_reduced(red + base + s + t, Kc[t], Vc[t], hist_out);
                 }
@@ -1101,8 +1139,9 @@ __global__ void k_bucket_long(const int64_t* __restrict__ cell_off, KV* __restri
     }
 }
 
-// Both CSRs from the reduced groups.  By input: a packed group is one
-// contiguous copy, an unpacked group copies cell by cell.  By output: each
+// Both CSRs from the reduced groups.  By input: the wave copies the group's
+// entries in order, each read from its row's offset (one contiguous range
+// for a packed group).  By output: each
 // entry goes to its output row at the row's offset plus its arrival rank
 // (counted by the reductions); k_rows_sort then orders each row by column,
 // which makes the result deterministic.  Also writes each pair's int32 row
@@ -1140,41 +1179,32 @@ __global__ void k_csr_groups(const PairCells* __restrict__ pcs,
     if (i0 >= i1) return;
     const int64_t* toff = t_off + P.out_gbase;
     const int64_t d0 = row_off[i0];
-    if (group_mode[grp] == 0) {
-        const int64_t b0 = cell_off[i0], cnt = row_off[i1] - d0;
-        // the row of entry q: the lanes' rows bracket the group's entries
-        const int64_t my_d = i < i1 ? row_off[i] - d0 : INT64_MAX;
-        for (int64_t q0 = 0; q0 < cnt; q0 += 64) {  // uniform trip count: shuffles below
-            const int64_t q = q0 + lane;
-            // row of entry q = the last lane whose first entry is at or before q
-            int32_t row = 0;
+    const int64_t cnt = row_off[i1] - d0;
+    // the row of entry q: the lanes' rows bracket the group's entries
+    const int64_t my_d = i < i1 ? row_off[i] - d0 : INT64_MAX;
+    // where the row's entries sit in `rec`, less the row's first entry: the
+    // group's start when packed, the row's own start when unpacked
+    const int64_t my_src = group_mode[grp] == 0 ? cell_off[i0]
+                           : i < i1             ? cell_off[i] - (row_off[i] - d0)
+                                                : 0;
+    for (int64_t q0 = 0; q0 < cnt; q0 += 64) {  // uniform trip count: shuffles below
+        const int64_t q = q0 + lane;
+        // row of entry q = the last lane whose first entry is at or before q
+        int32_t row = 0;
 #pragma unroll
-            for (int b = 32; b > 0; b >>= 1) {
-                const int64_t dd = __shfl(my_d, (row + b) & 63, 64);
-                if (row + b < 64 && dd <= q) row += b;
-            }
-            if (q < cnt) {
-                const KV r = rec[b0 + q];
-                const int32_t o = (int32_t)(uint32_t)r.key;
-                col[d0 + q] = o;
-                out_val[d0 + q] = r.val;
-                const int64_t pos = toff[o] + (int64_t)(r.key >> 32);
-                if (pos < nnz_cap) {
-                    t_col[pos] = (int32_t)(i0 + row - P.cell_gbase);
-                    t_val[pos] = r.val;
-                }
-            }
+        for (int b = 32; b > 0; b >>= 1) {
+            const int64_t dd = __shfl(my_d, (row + b) & 63, 64);
+            if (row + b < 64 && dd <= q) row += b;
         }
-    } else if (i < i1) {
-        const int64_t sb = cell_off[i], d = row_off[i], m = row_off[i + 1] - d;
-        for (int64_t q = 0; q < m; q++) {
-            const KV r = rec[sb + q];
+        const int64_t src = __shfl(my_src, row, 64) + q;
+        if (q < cnt) {
+            const KV r = rec[src];
             const int32_t o = (int32_t)(uint32_t)r.key;
-            col[d + q] = o;
-            out_val[d + q] = r.val;
+            col[d0 + q] = o;
+            out_val[d0 + q] = r.val;
             const int64_t pos = toff[o] + (int64_t)(r.key >> 32);
             if (pos < nnz_cap) {
-                t_col[pos] = (int32_t)(i - P.cell_gbase);
+                t_col[pos] = (int32_t)(i0 + row - P.cell_gbase);
                 t_val[pos] = r.val;
             }
         }
