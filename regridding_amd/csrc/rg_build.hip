@@ -794,6 +794,30 @@ __device__ inline void put_reduced(KV* __restrict__ dst, uint64_t out, double w,
     *dst = r;
 }
 
+// put_reduced for the m reduced entries of one cell (keys = output cells),
+// PUT_R rank atomics in flight at a time.
+constexpr int PUT_R = 8;
+
+__device__ inline void put_reduced_cell(KV* __restrict__ dst, const uint64_t* K,
+                                        const double* V, int64_t m,
+                                        int32_t* __restrict__ hist_out)
+{
+    for (int64_t t0 = 0; t0 < m; t0 += PUT_R) {
+        uint32_t rk[PUT_R];
+#pragma unroll
+        for (int u = 0; u < PUT_R; u++)
+            if (t0 + u < m) rk[u] = (uint32_t)atomicAdd(&hist_out[(int64_t)K[t0 + u]], 1);
+#pragma unroll
+        for (int u = 0; u < PUT_R; u++)
+            if (t0 + u < m) {
+                KV r;
+                r.key = K[t0 + u] | ((uint64_t)rk[u] << 32);
+                r.val = V[t0 + u];
+                dst[t0 + u] = r;
+            }
+    }
+}
+
 // The records of one group, reached through its four pass ranges.
 struct GroupRecs {
     int64_t ps0, ps1, ps2, ps3, e0, e1, e2;
@@ -936,7 +960,7 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
     } else {
         int32_t* hist_out =
             hist_out_all + pcs[__builtin_amdgcn_readfirstlane(pair_of_group[grp])].out_gbase;
-        for (int64_t t = 0; t < m; t++) put_reduced(red + base + dst + t, K[s + t], V[s + t], hist_out);
+        put_reduced_cell(red + base + dst, K + s, V + s, m, hist_out);
     }
     nnz_row[i] = (int32_t)m;
     if (lr) long_list[atomicAdd(n_long, 1)] = (int32_t)i;
@@ -1092,8 +1116,7 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_heavy(
                         red[base + s + t] = r;
                     }
                 } else {
-                    for (int64_t t = 0; t < m; t++)
-                        put_reduced(red + base + s + t, Kc[t], Vc[t], hist_out);
+                    put_reduced_cell(red + base + s, Kc, Vc, m, hist_out);
                 }
                 nnz_row[i] = (int32_t)m;
                 if (lrc) long_list[atomicAdd(n_long, 1)] = (int32_t)i;
