@@ -615,22 +615,44 @@ __device__ inline void wave_sync_lds()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Exclusive prefix over the wave of one value per lane.
+__device__ inline int32_t wave_exclusive(int32_t v, int lane)
+{
+    int32_t incl = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int32_t y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    return incl - v;
+}
+
 // Staged triples -> buckets, each tagged with its emission ordinal (the
 // chunk's first ordinal plus its position in the chunk) and its cell byte.
 // A wavefront takes its lanes' triples SCAT_R per lane per round and groups
 // them by bucket in an LDS hash table (one entry per bucket met in the
-// round: bucket, count, claimed base); one lane per entry claims the
-// bucket's records with a single atomic, and the wave writes every
-// bucket's records as one contiguous run.  A full table falls back to one
-// claim per triple.  A chunk's triples beyond the staging capacity come
-// from the overflow buffer (k_restage).
-constexpr int SCAT_R = 16;    // triples per lane per round
+// round: bucket, count, claimed base, offset in the round's staging); one
+// lane per entry claims the bucket's records with a single atomic.  The
+// round's records are then laid out in LDS bucket by bucket and written
+// out by consecutive lanes to consecutive addresses, so every bucket's run
+// goes to memory as whole lines rather than one 16-byte store per lane at
+// scattered addresses.  A full table falls back to one claim and one store
+// per triple.  A chunk's triples beyond the staging capacity come from the
+// overflow buffer (k_restage).
+constexpr int SCAT_R = 8;     // triples per lane per round
 constexpr int SCAT_HT = 256;  // hash entries per wavefront
+constexpr int SCAT_N = 64 * SCAT_R;
+constexpr int SCAT_BLOCK = 128;  // 2 wavefronts, 27 KiB of LDS per block
 
 struct ScatTable {
     int32_t key[SCAT_HT];
     int32_t cnt[SCAT_HT];
     uint32_t base[SCAT_HT];
+    int32_t off[SCAT_HT];
+    uint64_t rk[SCAT_N];  // the round's records, bucket by bucket
+    double rv[SCAT_N];
+    uint8_t rslot[SCAT_N];  // their table entry
+    uint8_t rc[SCAT_N];     // their cell byte
 };
 
 __global__ __launch_bounds__(WALK_MAX_BLOCK) void k_scatter(const Build* __restrict__ builds,
@@ -646,11 +668,11 @@ __global__ __launch_bounds__(WALK_MAX_BLOCK) void k_scatter(const Build* __restr
                           uint32_t n_rec,
                           int32_t* __restrict__ overflow)
 {
-    __shared__ ScatTable tables[WALK_MAX_BLOCK / 64];
+    extern __shared__ unsigned char scat_lds[];
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (t >= nthreads_all) return;  // nthreads_all is a multiple of 64: whole waves
     const int lane = threadIdx.x & 63;
-    ScatTable& H = tables[threadIdx.x >> 6];
+    ScatTable& H = ((ScatTable*)scat_lds)[threadIdx.x >> 6];
     const Build& B = build_of_thread(builds, pair_of_wave, t);
     const ChunkRef r = chunk_of_thread(B, t - B.thread_gbase);
     const int64_t g = B.chunk_gbase + r.g;
@@ -694,20 +716,56 @@ __global__ __launch_bounds__(WALK_MAX_BLOCK) void k_scatter(const Build* __restr
             }
         }
         wave_sync_lds();
-        for (int h = lane; h < SCAT_HT; h += 64)
-            if (H.key[h] >= 0) H.base[h] = atomicAdd(&cursor[H.key[h]], (uint32_t)H.cnt[h]);
+        // claims, and each entry's offset in the round's staging (a wave
+        // prefix over the table, SCAT_HT / 64 entries per lane)
+        int32_t c4[SCAT_HT / 64], sum = 0;
+#pragma unroll
+        for (int u = 0; u < SCAT_HT / 64; u++) {
+            const int h = lane * (SCAT_HT / 64) + u;
+            c4[u] = H.key[h] >= 0 ? H.cnt[h] : 0;
+            if (c4[u] > 0) H.base[h] = atomicAdd(&cursor[H.key[h]], (uint32_t)c4[u]);
+            sum += c4[u];
+        }
+        int32_t o = wave_exclusive(sum, lane);
+        const int32_t staged = __shfl(o + sum, 63, 64);
+#pragma unroll
+        for (int u = 0; u < SCAT_HT / 64; u++) {
+            H.off[lane * (SCAT_HT / 64) + u] = o;
+            o += c4[u];
+        }
         wave_sync_lds();
 #pragma unroll
         for (int k = 0; k < SCAT_R; k++) {
             if (slot[k] == -2) continue;
-            const uint32_t pos = slot[k] >= 0
-                                     ? H.base[slot[k]] + (uint32_t)local[k]
-                                     : atomicAdd(&cursor[e[k].cin >> 6], 1u);
-            if (pos < n_rec)
-                put_record(rec, rcell, pos, e[k].cin, e[k].cout, ord + (uint32_t)(q0 + k),
-                           e[k].a);
-            else
+            const uint64_t key = ((uint64_t)(uint32_t)e[k].cout << 32) | (uint64_t)(ord + (uint32_t)(q0 + k));
+            if (slot[k] >= 0) {
+                const int32_t i = H.off[slot[k]] + local[k];
+                H.rk[i] = key;
+                H.rv[i] = e[k].a;
+                H.rslot[i] = (uint8_t)slot[k];
+                H.rc[i] = (uint8_t)(e[k].cin & 63);
+            } else {
+                const uint32_t pos = atomicAdd(&cursor[e[k].cin >> 6], 1u);
+                if (pos < n_rec)
+                    put_record(rec, rcell, pos, e[k].cin, e[k].cout, ord + (uint32_t)(q0 + k),
+                               e[k].a);
+                else
+                    atomicAdd(overflow, 1);
+            }
+        }
+        wave_sync_lds();
+        for (int32_t i = lane; i < staged; i += 64) {
+            const int h = H.rslot[i];
+            const uint32_t pos = H.base[h] + (uint32_t)(i - H.off[h]);
+            if (pos < n_rec) {
+                KV rr;
+                rr.key = H.rk[i];
+                rr.val = H.rv[i];
+                rec[pos] = rr;
+                rcell[pos] = H.rc[i];
+            } else {
                 atomicAdd(overflow, 1);
+            }
         }
         wave_sync_lds();
         for (int h = lane; h < SCAT_HT; h += 64) {
@@ -838,18 +896,6 @@ struct GroupRecs {
                                                                : ps3 + (q - e2);
     }
 };
-
-// Exclusive prefix over the wave of one value per lane.
-__device__ inline int32_t wave_exclusive(int32_t v, int lane)
-{
-    int32_t incl = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int32_t y = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += y;
-    }
-    return incl - v;
-}
 
 // Groups whose records fit the LDS share; a larger group is listed for
 // k_reduce_heavy.
@@ -1907,7 +1953,8 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
             ov_tot, counters.p + 4);
         RG_HIP(hipGetLastError());
     }
-    k_scatter<<<grid_for(thr_tot, block), block, 0, s>>>(
+    k_scatter<<<grid_for(thr_tot, SCAT_BLOCK), SCAT_BLOCK, (SCAT_BLOCK / 64) * sizeof(ScatTable),
+                s>>>(
         d_builds, pair_of_wave.p, thr_tot, n_groups, count.p, stage.p, cap, ovbuf.p,
         ov_off.p, ov_tot, chunk_off.p, cursor, rec.p, rcell.p, (uint32_t)T, counters.p + 2);
     RG_HIP(hipGetLastError());
