@@ -49,9 +49,9 @@ namespace rg {
 // a sweep over the output grid; the left and right cells of an input edge),
 // so two pending (bucket, count) entries replace most atomics.
 struct HistCache {
-    int64_t b0 = -1, b1 = -1;
+    int32_t b0 = -1, b1 = -1;  // bucket indices (a build has fewer than 2^31 buckets)
     int32_t n0 = 0, n1 = 0;
-    __device__ void add(int32_t* __restrict__ hist, int64_t b, int32_t d)
+    __device__ void add(int32_t* __restrict__ hist, int32_t b, int32_t d)
     {
         if (b == b0) {
             n0 += d;
@@ -95,7 +95,7 @@ struct StageSink {
             stage[stage_index(g, n, cap)] = t;
         }
         n++;
-        hc.add(hist, p * ps + (cin >> 6), 1);
+        hc.add(hist, (int32_t)(p * ps + (cin >> 6)), 1);
     }
     __device__ void flush() { hc.flush(hist); }
 };
@@ -107,7 +107,7 @@ struct UncountSink {
     HistCache hc;
     __device__ void emit(int64_t cin, int64_t, double)
     {
-        hc.add(hist, p * ps + (cin >> 6), -1);
+        hc.add(hist, (int32_t)(p * ps + (cin >> 6)), -1);
     }
     __device__ void flush() { hc.flush(hist); }
 };
@@ -1853,6 +1853,7 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
 
     // ---- start states
     const int64_t n_groups = cell_tot / 64;  // every pair's cells padded to 64
+    if (4 * n_groups >= INT32_MAX) return fail(RG_ERR_ARGUMENT, "too many input cells in one build");
     DevBuf<int32_t> start, end, count, hist, counters;
     RG_TRY(start.alloc(chunk_tot, s));
     RG_TRY(end.alloc(chunk_tot, s));

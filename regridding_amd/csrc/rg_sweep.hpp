@@ -240,40 +240,38 @@ RG_HD bool box_touch(const Box& b, double xl, double yl, double xh, double yh)
 // p1 -> p2, ties resolved by scan order, ignoring the cell just exited.
 // Returns true and (cell, code, t) on a crossing.
 RG_HD bool outside_scan(const StaticGrid& s, double x1, double y1, double x2,
-                        double y2, int64_t skip_cell, int64_t* cell, int* code,
+                        double y2, int32_t skip_cell, int32_t* cell, int* code,
                         double* t_hit)
 {
-    double xl = x1 > x2 ? x2 : x1, xh = x1 > x2 ? x1 : x2;
-    double yl = y1 > y2 ? y2 : y1, yh = y1 > y2 ? y1 : y2;
+    // 32-bit indices (boundary edges of a grid below 2^31 vertices)
+    const double xl = x1 > x2 ? x2 : x1, xh = x1 > x2 ? x1 : x2;
+    const double yl = y1 > y2 ? y2 : y1, yh = y1 > y2 ? y1 : y2;
+    const int32_t nsup = (int32_t)s.nsup, nblk = (int32_t)s.nblk, nbe = (int32_t)s.nbe;
     double t_min = __builtin_huge_val();
-    bool found = false;
-    int64_t best = -1;
-    for (int64_t su = 0; su < s.nsup; su++) {
+    int32_t best = -1;
+    for (int32_t su = 0; su < nsup; su++) {
         if (!box_touch(s.bsup[su], xl, yl, xh, yh)) continue;
-        int64_t b1 = (su + 1) * BND_SUPER;
-        if (b1 > s.nblk) b1 = s.nblk;
-        for (int64_t bl = su * BND_SUPER; bl < b1; bl++) {
+        const int32_t b1 = min((su + 1) * BND_SUPER, nblk);
+        for (int32_t bl = su * BND_SUPER; bl < b1; bl++) {
             if (!box_touch(s.bblk[bl], xl, yl, xh, yh)) continue;
-            int64_t e1 = (bl + 1) * BND_BLOCK;
-            if (e1 > s.nbe) e1 = s.nbe;
-            for (int64_t e = bl * BND_BLOCK; e < e1; e++) {
+            const int32_t e1 = min((bl + 1) * BND_BLOCK, nbe);
+            for (int32_t e = bl * BND_BLOCK; e < e1; e++) {
                 const double* q = s.be + 4 * e;
                 TU r = seg_params(x1, y1, x2, y2, q[0], q[1], q[2], q[3]);
                 if (seg_hit(r) && r.t < t_min) {
-                    if ((int64_t)s.bcell[e] == skip_cell) continue;
-                    found = true;
+                    if (s.bcell[e] == skip_cell) continue;
                     t_min = r.t;
                     best = e;
                 }
             }
         }
     }
-    if (found) {
+    if (best >= 0) {
         *cell = s.bcell[best];
         *code = boundary_code(s, best);
         *t_hit = t_min;
     }
-    return found;
+    return best >= 0;
 }
 
 // A staged triple.  The count walk keeps up to `cap` triples per walk
@@ -321,7 +319,7 @@ RG_HD int32_t walk_chunk(const Build& B, const Pass& P, int64_t row64,
     const int32_t row = (int32_t)row64, kend = (int32_t)k1;
     const int32_t ncy_sweep = (int32_t)P.ncy_sweep, nrows = (int32_t)P.nrows;
     const int32_t sk = (int32_t)P.sk;
-    const int64_t limit = step_limit(s, k1 - k0);
+    const int32_t limit = (int32_t)min(step_limit(s, k1 - k0), (int64_t)INT32_MAX - 1);
     int32_t k = (int32_t)k0;
     int32_t widx = row * (int32_t)P.sr + k * sk;  // sweep vertex (row, k)
     double x1 = P.xs[widx], y1 = P.ys[widx];
@@ -329,7 +327,7 @@ RG_HD int32_t walk_chunk(const Build& B, const Pass& P, int64_t row64,
     int32_t sx = outside ? 0 : start / ncy;
     int32_t sy = outside ? 0 : start % ncy;
     int32_t edge_last = -1;  // sys.maxsize
-    int64_t steps = 0;
+    int32_t steps = 0;
     int32_t csx = -2, csy = -2;  // cell whose corners are cached
     double2 c0, c1, c2, c3;
     c0.x = c0.y = c1.x = c1.y = c2.x = c2.y = c3.x = c3.y = 0;
@@ -337,14 +335,17 @@ RG_HD int32_t walk_chunk(const Build& B, const Pass& P, int64_t row64,
         if (++steps > limit) return STATE_FAILED;
         double x2 = P.xs[widx + sk], y2 = P.ys[widx + sk];
         if (outside) {
-            int64_t cell;
+            int32_t cell;
             int code;
             double t;
+            // the cached corners are reloaded after an outside stretch
+            csx = csy = -2;
+            c0.x = c0.y = c1.x = c1.y = c2.x = c2.y = c3.x = c3.y = 0;
             if (outside_scan(s, x1, y1, x2, y2, edge_last, &cell, &code, &t)) {
                 x2 = x1 + t * (x2 - x1);
                 y2 = y1 + t * (y2 - y1);
-                sx = (int32_t)cell / ncy;
-                sy = (int32_t)cell % ncy;
+                sx = cell / ncy;
+                sy = cell % ncy;
                 edge_last = code;
                 outside = false;
             } else {
