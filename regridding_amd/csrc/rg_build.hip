@@ -1377,12 +1377,28 @@ __global__ void k_pair_bases(const PairCells* __restrict__ pcs,
                              const Build* __restrict__ builds, int64_t npairs,
                              const int64_t* __restrict__ row_off, int64_t ncells_all,
                              const int64_t* __restrict__ chunk_off,
-                             int64_t nchunks_all, int64_t* __restrict__ out)
+                             int64_t nchunks_all, const int32_t* __restrict__ counters,
+                             int64_t* __restrict__ out)
 {
     const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k > npairs) return;
     out[2 * k] = row_off[k < npairs ? pcs[k].cell_gbase : ncells_all];
     out[2 * k + 1] = chunk_off[k < npairs ? builds[k].chunk_gbase : nchunks_all];
+    if (k == npairs) {  // the eight counters after the bases: one read for all
+        int32_t* c = (int32_t*)(out + 2 * (npairs + 1));
+        for (int q = 0; q < 8; q++) c[q] = counters[q];
+    }
+}
+
+// The triple total, the histogram total and the eight counters in one
+// buffer, read back with one copy.
+__global__ void k_pack_totals(const int64_t* __restrict__ a, const int64_t* __restrict__ b,
+                              const int32_t* __restrict__ counters, int64_t* __restrict__ out)
+{
+    out[0] = *a;
+    out[1] = *b;
+    int32_t* c = (int32_t*)(out + 2);
+    for (int q = 0; q < 8; q++) c[q] = counters[q];
 }
 
 __global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n)
@@ -1910,9 +1926,16 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
         T = caps->T_cap;
         ov_tot = caps->ov_cap;
     } else {
-        RG_TRY(read_scalars(chunk_off.p + chunk_tot, &tot[0], sizeof(int64_t), s));
-        RG_TRY(read_scalars(bucket_off.p + 4 * n_groups, &tot[1], sizeof(int64_t), s));
-        RG_TRY(read_scalars(counters.p, cnt, sizeof(cnt), s));
+        DevBuf<int64_t> packed;
+        RG_TRY(packed.alloc(6, s));
+        k_pack_totals<<<1, 1, 0, s>>>(chunk_off.p + chunk_tot, bucket_off.p + 4 * n_groups,
+                                      counters.p, packed.p);
+        RG_HIP(hipGetLastError());
+        int64_t hp[6];
+        RG_TRY(read_scalars(packed.p, hp, sizeof(hp), s));
+        tot[0] = hp[0];
+        tot[1] = hp[1];
+        std::memcpy(cnt, hp + 2, sizeof(cnt));
         std::memcpy(&ov_tot, cnt + 6, sizeof(ov_tot));
         up.pending = false;  // the stream has passed the upload
         if (cnt[2] != 0)
@@ -2006,11 +2029,12 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
     RG_TRY(scan_i32_to_i64(nnz_row.p, row_off.p, cell_tot, s));
     // per-pair first entry and first triple (slot npairs: totals)
     DevBuf<int64_t> bases;
-    RG_TRY(bases.alloc(2 * (npairs + 1), s));
+    RG_TRY(bases.alloc(2 * (npairs + 1) + 4, s));
     k_pair_bases<<<grid_for(npairs + 1, 256), 256, 0, s>>>(
-        d_pcs, d_builds, npairs, row_off.p, cell_tot, chunk_off.p, chunk_tot, bases.p);
+        d_pcs, d_builds, npairs, row_off.p, cell_tot, chunk_off.p, chunk_tot, counters.p,
+        bases.p);
     RG_HIP(hipGetLastError());
-    std::vector<int64_t> hbases(2 * (npairs + 1));
+    std::vector<int64_t> hbases(2 * (npairs + 1) + 4);
     int32_t cnt2[8] = {};
     int64_t nnz;
     if (capmode) {
@@ -2019,7 +2043,7 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
         nnz = caps->nnz_cap;
     } else {
         RG_TRY(read_scalars(bases.p, hbases.data(), sizeof(int64_t) * hbases.size(), s));
-        RG_TRY(read_scalars(counters.p, cnt2, sizeof(cnt2), s));
+        std::memcpy(cnt2, hbases.data() + 2 * (npairs + 1), sizeof(cnt2));
         if (cnt2[2] != 0)
             return fail(RG_ERR_HIP, "internal: the emission did not match the "
                                     "counted buckets (" + std::to_string(cnt2[2]) +

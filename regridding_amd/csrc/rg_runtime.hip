@@ -321,11 +321,15 @@ __global__ void k_scan_tiles(const T* __restrict__ in, int64_t* __restrict__ out
         if (i < n) out[i] = run;
         run += v[q];
     }
-    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+    if (threadIdx.x == 0) {
+        tile_sums[blockIdx.x] = total;
+        if (gridDim.x == 1) out[n] = total;  // one tile: its sum is the total
+    }
 }
 
+// Adds each tile's offset; block 0 also writes the total (out[n]).
 __global__ void k_scan_add(int64_t* __restrict__ out,
-                           const int64_t* __restrict__ tile_off, int64_t n)
+                           const int64_t* __restrict__ tile_off, int64_t n, int64_t ntiles)
 {
     const int64_t tile0 = (int64_t)blockIdx.x * SCAN_TILE;
     const int64_t add = tile_off[blockIdx.x];
@@ -333,20 +337,10 @@ __global__ void k_scan_add(int64_t* __restrict__ out,
         int64_t i = tile0 + q;
         if (i < n) out[i] += add;
     }
-}
-
-__global__ void k_set_total(int64_t* out, int64_t n, const int64_t* tile_off,
-                            int64_t ntiles)
-{
-    out[n] = tile_off[ntiles];
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = tile_off[ntiles];
 }
 
 __global__ void k_zero_total(int64_t* out) { out[0] = 0; }
-
-__global__ void k_copy_total(int64_t* out, int64_t n, const int64_t* tile_sums)
-{
-    out[n] = tile_sums[0];
-}
 
 template <class T>
 int scan_impl(const T* in, int64_t* out, int64_t n, hipStream_t s)
@@ -362,15 +356,9 @@ int scan_impl(const T* in, int64_t* out, int64_t n, hipStream_t s)
     RG_TRY(offs.alloc(ntiles + 1, s));
     k_scan_tiles<T><<<(unsigned)ntiles, SCAN_BLOCK, 0, s>>>(in, out, sums.p, n);
     RG_HIP(hipGetLastError());
-    if (ntiles == 1) {  // one tile: its sum is the total, nothing to add
-        k_copy_total<<<1, 1, 0, s>>>(out, n, sums.p);
-        RG_HIP(hipGetLastError());
-        return RG_OK;
-    }
+    if (ntiles == 1) return RG_OK;  // the tile wrote the total
     RG_TRY(scan_impl<int64_t>(sums.p, offs.p, ntiles, s));
-    k_scan_add<<<(unsigned)ntiles, SCAN_BLOCK, 0, s>>>(out, offs.p, n);
-    RG_HIP(hipGetLastError());
-    k_set_total<<<1, 1, 0, s>>>(out, n, offs.p, ntiles);
+    k_scan_add<<<(unsigned)ntiles, SCAN_BLOCK, 0, s>>>(out, offs.p, n, ntiles);
     RG_HIP(hipGetLastError());
     return RG_OK;
 }
