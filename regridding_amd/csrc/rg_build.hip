@@ -213,6 +213,7 @@ struct GridDesc {
     int64_t blk_base;   // first edge block box
     int64_t sup_base;   // first super-block box
     int64_t tile_base;  // first vertex tile
+    int64_t pyr_base;   // first polygon-edge run (y range)
 };
 
 // The cells of one pair of a batched build.
@@ -320,6 +321,37 @@ __global__ void k_grid_boundary(const GridDesc* __restrict__ grids,
     boundary_edge(G.x, G.y, G.nx, G.ny, le, be + 4 * e, bcell + e);
 }
 
+// The y range of the vertices of each run of 64 boundary-polygon edges
+// (edge k runs from vertex k - 1 to vertex k, edge 0 from the last vertex):
+// a run whose range excludes a point's y adds nothing to its winding number.
+__global__ void k_grid_poly_yranges(const GridDesc* __restrict__ grids,
+                                    const int64_t* __restrict__ pyr_prefix, int ngrids,
+                                    int64_t npyr_all, const double* __restrict__ bpy_all,
+                                    double2* __restrict__ pyr)
+{
+    // one wavefront per run, one lane per edge
+    const int64_t b = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (b >= npyr_all) return;  // uniform per wavefront
+    const GridDesc& G = grids[seg_of(pyr_prefix, ngrids, b)];
+    const int64_t nbp = 2 * (G.nx - 1) + 2 * (G.ny - 1);
+    const int64_t k = (b - G.pyr_base) * 64 + lane;
+    const double* bpy = bpy_all + G.e_base;
+    double lo = __builtin_huge_val(), hi = -__builtin_huge_val();
+    if (k < nbp) {
+        const int64_t km = k == 0 ? nbp - 1 : k - 1;
+        // fmin/fmax ignore NaN; an edge with a NaN end adds nothing
+        lo = fmin(bpy[km], bpy[k]);
+        hi = fmax(bpy[km], bpy[k]);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = fmin(lo, __shfl_xor(lo, o, 64));
+        hi = fmax(hi, __shfl_xor(hi, o, 64));
+    }
+    if (lane == 0) pyr[b] = make_double2(lo, hi);
+}
+
 __global__ void k_grid_edge_boxes(const GridDesc* __restrict__ grids,
                                   const int64_t* __restrict__ blk_prefix,
                                   int ngrids, int64_t nblk_all,
@@ -419,10 +451,26 @@ __global__ void k_row_start(const Build* __restrict__ builds,
     int32_t state = STATE_OUTSIDE;
     if (s.bbox[0] <= px && px <= s.bbox[2] && s.bbox[1] <= py &&
         py <= s.bbox[3]) {
+        // the winding number: a sum of +-1 and +-0.5 terms, exact in any
+        // order; runs of 64 edges whose vertices all lie strictly above or
+        // below py add nothing (their y - py never changes sign or is zero)
         double w = 0;
-        for (int64_t k = lane; k < s.nbp; k += 64) {
-            int64_t km = k == 0 ? s.nbp - 1 : k - 1;
-            w += winding_term(px, py, s.bpx[km], s.bpy[km], s.bpx[k], s.bpy[k]);
+        for (int64_t b0 = 0; b0 < s.npyr; b0 += 64) {  // uniform
+            const int64_t b = b0 + lane;
+            bool touch = false;
+            if (b < s.npyr) {
+                const double2 r = s.pyr[b];
+                touch = !(py < r.x || py > r.y);
+            }
+            uint64_t m = __ballot(touch);
+            while (m) {
+                const int64_t k = (b0 + __builtin_ctzll(m)) * 64 + lane;
+                m &= m - 1;
+                if (k < s.nbp) {
+                    const int64_t km = k == 0 ? s.nbp - 1 : k - 1;
+                    w += winding_term(px, py, s.bpx[km], s.bpy[km], s.bpx[k], s.bpy[k]);
+                }
+            }
         }
         w = wave_sum(w);
         if (w != 0) {
@@ -1627,8 +1675,9 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
     // ---- host tables: grids (2k input, 2k+1 output), cells, passes
     const int ng = (int)(2 * npairs);
     std::vector<GridDesc> grids(ng);
-    std::vector<int64_t> tile_pf(ng + 1), e_pf(ng + 1), blk_pf(ng + 1), sup_pf(ng + 1);
-    int64_t v_tot = 0, e_tot = 0, blk_tot = 0, sup_tot = 0, tile_tot = 0;
+    std::vector<int64_t> tile_pf(ng + 1), e_pf(ng + 1), blk_pf(ng + 1), sup_pf(ng + 1),
+        pyr_pf(ng + 1);
+    int64_t v_tot = 0, e_tot = 0, blk_tot = 0, sup_tot = 0, tile_tot = 0, pyr_tot = 0;
     for (int g = 0; g < ng; g++) {
         const rg_pair_desc& d = pairs[g / 2];
         GridDesc& G = grids[g];
@@ -1642,6 +1691,8 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
         G.v_base = v_tot;
         G.e_base = e_pf[g] = e_tot;
         G.blk_base = blk_pf[g] = blk_tot;
+        G.pyr_base = pyr_pf[g] = pyr_tot;
+        pyr_tot += (nbe + 63) / 64;
         G.sup_base = sup_pf[g] = sup_tot;
         G.tile_base = tile_pf[g] = tile_tot;
         v_tot += nv;
@@ -1653,12 +1704,14 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
     tile_pf[ng] = tile_tot;
     e_pf[ng] = e_tot;
     blk_pf[ng] = blk_tot;
+    pyr_pf[ng] = pyr_tot;
     sup_pf[ng] = sup_tot;
 
     DevBuf<double2> xy_all;
     DevBuf<double> part, bbox_all, bpx, bpy, be;
     DevBuf<int32_t> bcell;
     DevBuf<Box> bblk, bsup;
+    DevBuf<double2> pyr;
     RG_TRY(xy_all.alloc(v_tot, s));
     RG_TRY(part.alloc(4 * tile_tot, s));
     RG_TRY(bbox_all.alloc(4 * ng, s));
@@ -1667,6 +1720,7 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
     RG_TRY(be.alloc(4 * e_tot, s));
     RG_TRY(bcell.alloc(e_tot, s));
     RG_TRY(bblk.alloc(blk_tot, s));
+    RG_TRY(pyr.alloc(pyr_tot, s));
     RG_TRY(bsup.alloc(sup_tot, s));
     std::vector<StaticGrid> sg(ng);
     for (int g = 0; g < ng; g++) {
@@ -1686,6 +1740,8 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
         S.nbe = S.nbp;
         S.bblk = bblk.p + G.blk_base;
         S.nblk = blk_pf[g + 1] - blk_pf[g];
+        S.pyr = pyr.p + G.pyr_base;
+        S.npyr = pyr_pf[g + 1] - pyr_pf[g];
         S.bsup = bsup.p + G.sup_base;
         S.nsup = sup_pf[g + 1] - sup_pf[g];
     }
@@ -1769,6 +1825,7 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
     const size_t o_tile = append_table(hb, tile_pf.data(), tile_pf.size());
     const size_t o_e = append_table(hb, e_pf.data(), e_pf.size());
     const size_t o_blk = append_table(hb, blk_pf.data(), blk_pf.size());
+    const size_t o_pyr = append_table(hb, pyr_pf.data(), pyr_pf.size());
     const size_t o_sup = append_table(hb, sup_pf.data(), sup_pf.size());
     const size_t o_wave = append_table(hb, wave_pf.data(), wave_pf.size());
     const size_t o_row = append_table(hb, row_pf.data(), row_pf.size());
@@ -1828,6 +1885,7 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
     const int64_t* d_tile = (const int64_t*)(meta.p + o_tile);
     const int64_t* d_e = (const int64_t*)(meta.p + o_e);
     const int64_t* d_blk = (const int64_t*)(meta.p + o_blk);
+    const int64_t* d_pyr = (const int64_t*)(meta.p + o_pyr);
     const int64_t* d_sup = (const int64_t*)(meta.p + o_sup);
 
     // ---- prepare
@@ -1855,6 +1913,9 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
     RG_HIP(hipGetLastError());
     k_grid_edge_boxes<<<grid_for(blk_tot, 256), 256, 0, s>>>(d_grids, d_blk, ng,
                                                              blk_tot, be.p, bblk.p);
+    RG_HIP(hipGetLastError());
+    k_grid_poly_yranges<<<grid_for(pyr_tot * 64, 256), 256, 0, s>>>(d_grids, d_pyr, ng,
+                                                                    pyr_tot, bpy.p, pyr.p);
     RG_HIP(hipGetLastError());
     k_grid_super_boxes<<<grid_for(sup_tot, 256), 256, 0, s>>>(d_grids, d_sup, ng,
                                                               sup_tot, bblk.p, bsup.p);

@@ -55,6 +55,8 @@ struct StaticGrid {
     int64_t nblk;
     const Box* bsup;         // box per BND_SUPER blocks
     int64_t nsup;
+    const double2* pyr;      // y range (min, max) of the vertices of each
+    int64_t npyr;            // run of 64 boundary-polygon edges
 };
 
 // One (pass, axis) sweep: rows of the aligned sweep grid walked across the

@@ -6,6 +6,7 @@
 // decomposition and order of phases as rg_build.hip.  The tests compare its
 // emission sequence and coalesced weights with the oracle bit for bit, which
 // checks the chunk decomposition and every device function without a GPU.
+#include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -452,6 +453,33 @@ int64_t rge_layout_trace(const int64_t* ops, int64_t n, int64_t* off)
         }
     }
     return (int64_t)L.peak;
+}
+
+// The row start's winding number over a boundary polygon, by every edge
+// (skip = 0) or skipping the runs of 64 edges whose vertex y range excludes
+// py, as k_row_start does (skip = 1).
+double rge_winding(const double* bpx, const double* bpy, int64_t nbp, double px, double py,
+                   int skip)
+{
+    double w = 0;
+    const int64_t nb = (nbp + 63) / 64;
+    for (int64_t b = 0; b < nb; b++) {
+        const int64_t k1 = (b + 1) * 64 < nbp ? (b + 1) * 64 : nbp;
+        if (skip) {
+            double lo = __builtin_huge_val(), hi = -__builtin_huge_val();
+            for (int64_t k = b * 64; k < k1; k++) {
+                const int64_t km = k == 0 ? nbp - 1 : k - 1;
+                lo = fmin(lo, fmin(bpy[km], bpy[k]));
+                hi = fmax(hi, fmax(bpy[km], bpy[k]));
+            }
+            if (py < lo || py > hi) continue;
+        }
+        for (int64_t k = b * 64; k < k1; k++) {
+            const int64_t km = k == 0 ? nbp - 1 : k - 1;
+            w += winding_term(px, py, bpx[km], bpy[km], bpx[k], bpy[k]);
+        }
+    }
+    return w;
 }
 
 }  // extern "C"
