@@ -33,7 +33,7 @@ class BuildApplyGraph:
 
     def __init__(self, pairs: PackedPairs, values=None, slack: float = 0.25,
                  edges_per_lane: int = 0, block: int = 0, stream: int = 0,
-                 device_jobs=None):
+                 device_jobs=None, stage_per_edge: int = 0):
         self.pairs = pairs
         L = lib()
         n = pairs.n
@@ -42,7 +42,7 @@ class BuildApplyGraph:
             d = pairs.desc[k]
             self.n_in.append((d.nxi - 1) * (d.nyi - 1))
             self.n_out.append((d.nxo - 1) * (d.nyo - 1))
-        t = rg_tuning(int(edges_per_lane), int(block), 0, 0)
+        t = rg_tuning(int(edges_per_lane), int(block), 0, int(stage_per_edge))
         jobs = (rg_graph_job * n)()
         self.stream = stream
         if device_jobs is not None:

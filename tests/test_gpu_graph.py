@@ -96,6 +96,23 @@ def test_graph_replay_after_in_place_update():
         g.close()
 
 
+def test_graph_with_overflowing_staging():
+    # one staged triple per edge: most chunks keep their triples in the
+    # overflow buffer, sized by capacity like the others
+    pairs = _pairs()
+    packed = pack_pairs(pairs)
+    vals = _values(pairs, 6)
+    g = BuildApplyGraph(packed, vals, slack=0.25, stage_per_edge=1)
+    try:
+        assert g.capacity()["overflow"] > 4096
+        for rep in range(2):
+            g.launch()
+            assert g.status() == 0
+            _check_replay(g, pairs, vals)
+    finally:
+        g.close()
+
+
 def _shifted(pairs, dx):
     return [(ci, (co[0] + dx, co[1])) for ci, co in pairs]
 
@@ -108,7 +125,8 @@ def test_graph_capacity_overrun_is_flagged():
     apart = _shifted(pairs, 10.0)
     packed = pack_pairs(apart)
     vals = _values(pairs, 5)
-    g = BuildApplyGraph(packed, vals, slack=0.0)
+    # (one staged triple per edge: the replay also outgrows the overflow buffer)
+    g = BuildApplyGraph(packed, vals, slack=0.0, stage_per_edge=1)
     try:
         g.launch()
         assert g.status() == 0
