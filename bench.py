@@ -712,6 +712,34 @@ def cpu_baseline(ci, co, values, handle) -> dict:
     log(f"cpu oracle ({cores} threads): sweep {t1 - t0:.2f} s, coalesce {t2 - t1:.2f} s, "
         f"apply {t3 - t2:.2f} s; GPU weights bit-identical: {parity}")
     del y
+    # the small CPU-runnable pair on one thread and on all: the scalar figure
+    from regridding_amd import _util, workloads
+
+    sgi, sgo, svals = workloads.plain_small()
+    sci, sco, *_ = _util.normalize_coordinates(sgi, sgo, perturb=True)
+    small = {}
+    for nt in (1, cores):
+        oracle.set_threads(nt)
+        best = None
+        for _ in range(3):
+            u0 = time.perf_counter()
+            sii, sio, sw = oracle.coalesce(*oracle.sweep_2d(sci, sco))
+            oracle.apply(sii, sio, sw, svals.reshape(svals.shape[0], -1),
+                         (sco[0].shape[0] - 1) * (sco[0].shape[1] - 1))
+            dt = time.perf_counter() - u0
+            best = dt if best is None else min(best, dt)
+        small[f"seconds_{nt}_threads"] = round(best, 4)
+    oracle.set_threads(cores)
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    log(f"cpu oracle, 129x129 -> 101x101 pair: {small}")
     return {
         "value": round(1.0 / (t3 - t0), 6),
         "unit": "grid-pairs/s",
@@ -721,6 +749,9 @@ def cpu_baseline(ci, co, values, handle) -> dict:
                   "(image-parallel), one run",
         "seconds": {"sweep": round(t1 - t0, 3), "coalesce": round(t2 - t1, 3), "apply": round(t3 - t2, 3)},
         "gpu_weights_bit_identical": parity,
+        "cpu_model": cpu_model,
+        "small_pair": dict(small, sample="129x129 -> 101x101 pair, 4 images: sweep + coalesce + "
+                                         "apply, best of 3"),
     }
 
 
