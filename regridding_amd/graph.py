@@ -27,8 +27,12 @@ class BuildApplyGraph:
     """
     ``pairs``: a ``PackedPairs`` (kept alive; its buffer is what replays
     read).  ``values``: per pair a (batch, n_in) float64 host array (batch
-    may be 0).  ``slack``: extra room over the measured triple and weight
-    counts, for replays whose coordinates produce more of them.
+    may be 0; None: build only), copied to the device once; or
+    ``device_jobs``: per pair (input pointer, output pointer, batch) of
+    device arrays the caller owns.  ``slack``: extra room over the measured
+    triple and weight counts, for replays whose coordinates produce more of
+    them.  ``edges_per_lane``, ``block``, ``stage_per_edge``: the build's
+    launch shape and staging (0: defaults).
     """
 
     def __init__(self, pairs: PackedPairs, values=None, slack: float = 0.25,
@@ -49,11 +53,17 @@ class BuildApplyGraph:
             # (input pointer, output pointer, batch) per pair: device arrays
             # the caller owns and keeps alive while the graph exists
             self.x = self.y = None
+            if len(device_jobs) != n:
+                raise ValueError(f"{len(device_jobs)} jobs for {n} pairs")
             for k, (xp, yp, b) in enumerate(device_jobs):
                 jobs[k].values_input, jobs[k].values_output, jobs[k].batch = xp, yp, int(b)
                 self.batch.append(int(b))
             self._create(pairs, t, jobs, slack, stream)
             return
+        if values is None:  # build only
+            values = [None] * n
+        if len(values) != n:
+            raise ValueError(f"{len(values)} value arrays for {n} pairs")
         x_parts, self._x_off, self._y_off = [], [], []
         xo = yo = 0
         for k, v in enumerate(values):
