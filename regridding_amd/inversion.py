@@ -13,12 +13,25 @@ from __future__ import annotations
 
 import torch
 
-from .torch_ops import regrid_apply
+from .torch_ops import _apply_views, regrid_apply, regrid_apply_views
 
 __all__ = ["lipschitz", "least_squares"]
 
 
-def _loss(views, x):
+def _fused(views):
+    """The views' weights and concatenated observations when every view
+    takes the same input cells (one batched launch per direction), else None."""
+    ws = [w for w, _ in views]
+    if len(ws) < 2 or any(w.n_in != ws[0].n_in for w in ws):
+        return None
+    return tuple(ws), torch.cat([y.reshape(-1) for _, y in views])
+
+
+def _loss(views, x, fused=None):
+    if fused is not None:
+        ws, ycat = fused
+        r = regrid_apply_views(ws, x) - ycat
+        return 0.5 * (r * r).sum()
     total = x.new_zeros(())
     for w, y in views:
         r = regrid_apply(w, x) - y
@@ -32,10 +45,15 @@ def lipschitz(views, n: int, iters: int = 30, device=None, dtype=torch.float64) 
     v = torch.ones(n, dtype=dtype, device=device)
     v /= v.norm()
     lam = 0.0
+    fused = _fused(views)
     for _ in range(iters):
-        u = torch.zeros_like(v)
-        for w, _y in views:
-            u += regrid_apply(w, regrid_apply(w, v), transpose=True)
+        if fused is not None:
+            ws = fused[0]
+            u = _apply_views(ws, _apply_views(ws, v, False), True).sum(0)
+        else:
+            u = torch.zeros_like(v)
+            for w, _y in views:
+                u += regrid_apply(w, regrid_apply(w, v), transpose=True)
         lam = float(u.norm())
         v = u / lam
     return lam
@@ -50,17 +68,18 @@ def least_squares(views, x0: torch.Tensor, iters: int = 200, step: float | None 
     x = x0.detach().clone()
     z = x.clone()
     t = 1.0
-    losses = []
+    fused = _fused(views)
+    losses = []  # device scalars: no host wait inside the loop
     for _ in range(iters):
         z.requires_grad_(True)
-        loss = _loss(views, z)
+        loss = _loss(views, z, fused)
         (g,) = torch.autograd.grad(loss, z)
-        losses.append(float(loss.detach()))
+        losses.append(loss.detach())
         with torch.no_grad():
             x_new = z - step * g
             t_new = 0.5 * (1.0 + (1.0 + 4.0 * t * t) ** 0.5)
             z = x_new + ((t - 1.0) / t_new) * (x_new - x)
             x, t = x_new, t_new
     with torch.no_grad():
-        losses.append(float(_loss(views, x)))
-    return x, losses
+        losses.append(_loss(views, x, fused))
+    return x, [float(v) for v in torch.stack(losses).cpu()]

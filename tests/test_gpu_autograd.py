@@ -14,7 +14,7 @@ from oracle import oracle
 from regridding_amd import _util, workloads
 from regridding_amd._conservative import build_2d, build_2d_batched
 from regridding_amd.inversion import least_squares, lipschitz
-from regridding_amd.torch_ops import regrid_apply
+from regridding_amd.torch_ops import regrid_apply, regrid_apply_views
 
 pytestmark = pytest.mark.gpu
 
@@ -49,6 +49,50 @@ def test_backward_is_transpose_weights_apply():
     # transpose_weights swaps the index arrays; applying them accumulates
     # every input cell in ascending output order
     assert np.array_equal(x.grad.cpu().numpy(), oracle.apply(io, ii, ww, gy, w.n_in))
+
+
+def _views(n_views=3, n_in=33):
+    gi = workloads.warped_grid(n_in, 0.3, 0.05, 0.05)
+    pairs = []
+    for k in range(n_views):
+        go = workloads.warped_grid(21 + 6 * k, 0.1 * k, 0.02, 0.0, -0.9, 0.9)
+        ci, co, *_ = _util.normalize_coordinates(gi, go, perturb=True)
+        pairs.append((ci, co))
+    return build_2d_batched(pairs)
+
+
+def test_views_forward_and_backward_match_single_views():
+    ws = _views()
+    rng = np.random.default_rng(11)
+    x = torch.tensor(rng.normal(size=ws[0].n_in), device="cuda", requires_grad=True)
+    y = regrid_apply_views(ws, x)
+    ref = torch.cat([regrid_apply(w, x.detach()) for w in ws])
+    assert torch.equal(y.detach(), ref)  # the same gather per element, bit for bit
+    g = torch.tensor(rng.normal(size=y.numel()), device="cuda")
+    y.backward(g)
+    parts = torch.split(g, [w.n_out for w in ws])
+    adj = torch.stack([regrid_apply(w, p, transpose=True) for w, p in zip(ws, parts)])
+    # each element's adjoint is bit-identical; the sum over elements is torch's
+    assert torch.equal(x.grad, adj.sum(0))
+
+
+def test_views_gradcheck_fp64():
+    ws = _views(2, 17)
+    x = torch.randn(ws[0].n_in, dtype=torch.float64, device="cuda", requires_grad=True)
+    assert torch.autograd.gradcheck(lambda v: regrid_apply_views(ws, v), (x,), eps=1e-6,
+                                    atol=1e-9)
+
+
+def test_views_fp32_and_errors():
+    ws = _views()
+    x = torch.randn(ws[0].n_in, dtype=torch.float64, device="cuda")
+    out = regrid_apply_views(ws, x.float())
+    ref = regrid_apply_views(ws, x).float()
+    assert torch.max(torch.abs(out - ref)) <= 1e-5 * torch.max(torch.abs(ref))
+    with pytest.raises(ValueError):
+        regrid_apply_views(ws, x[:-1])
+    with pytest.raises(ValueError):
+        regrid_apply_views(ws, x.reshape(1, -1))
 
 
 def test_adjoint_identity_and_fp32():
@@ -86,6 +130,13 @@ def test_inversion_reduces_misfit_and_error():
     x0 = torch.full_like(truth, float(np.mean(image)))
     x, losses = least_squares(views_t, x0, iters=60)
     assert losses[-1] < 0.05 * losses[0]
+    # the one-launch loss of all views equals the sum of the per-view losses
+    from regridding_amd.inversion import _fused, _loss
+
+    fused = _fused(views_t)
+    assert fused is not None
+    a, b = float(_loss(views_t, x, fused)), float(_loss(views_t, x, None))
+    assert abs(a - b) <= 1e-12 * abs(b)
     err0 = float(torch.linalg.norm(x0 - truth))
     err = float(torch.linalg.norm(x - truth))
     assert err < 0.5 * err0
