@@ -1775,6 +1775,7 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
             const int64_t nys = sweep_input ? d.nyi : d.nyo;
             Q.xs = sweep_input ? d.xi : d.xo;
             Q.ys = sweep_input ? d.yi : d.yo;
+            Q.xys = sweep_input ? sg[2 * k].xy : sg[2 * k + 1].xy;
             if (axis == 0) {  // aligned view = transpose
                 Q.sr = 1; Q.sk = nys; Q.nrows = nys; Q.nk = nxs;
             } else {

@@ -64,6 +64,7 @@ struct StaticGrid {
 struct Pass {
     const double* xs;        // sweep grid vertices, original orientation
     const double* ys;
+    const double2* xys;      // the same vertices interleaved (one load each)
     int64_t sr, sk;          // element (row, k) of the aligned view
     int64_t nrows, nk;       // aligned view shape
     int64_t ncy_sweep;       // cells along the last axis of the sweep grid
@@ -325,6 +326,10 @@ RG_HD int32_t walk_chunk(const Build& B, const Pass& P, int64_t row64,
     int32_t k = (int32_t)k0;
     int32_t widx = row * (int32_t)P.sr + k * sk;  // sweep vertex (row, k)
     double x1 = P.xs[widx], y1 = P.ys[widx];
+    // the end of the current sweep edge, loaded when the walk reaches a
+    // new edge (a crossing keeps it)
+    double2 ev = make_double2(0, 0);
+    if (k < kend) ev = P.xys[widx + sk];
     bool outside = start < 0;
     int32_t sx = outside ? 0 : start / ncy;
     int32_t sy = outside ? 0 : start % ncy;
@@ -335,7 +340,7 @@ RG_HD int32_t walk_chunk(const Build& B, const Pass& P, int64_t row64,
     c0.x = c0.y = c1.x = c1.y = c2.x = c2.y = c3.x = c3.y = 0;
     while (k < kend) {
         if (++steps > limit) return STATE_FAILED;
-        double x2 = P.xs[widx + sk], y2 = P.ys[widx + sk];
+        double x2 = ev.x, y2 = ev.y;
         if (outside) {
             int32_t cell;
             int code;
@@ -354,6 +359,7 @@ RG_HD int32_t walk_chunk(const Build& B, const Pass& P, int64_t row64,
                 k += 1;
                 widx += sk;
                 edge_last = -1;
+                if (k < kend) ev = P.xys[widx + sk];
             }
         } else {
             // _step_inside_static (:561-749).  Cell corners c0..c3 =
@@ -433,7 +439,10 @@ RG_HD int32_t walk_chunk(const Build& B, const Pass& P, int64_t row64,
                 sx = nsx;
                 sy = nsy;
             }
-            if (k_next != k) widx += sk;
+            if (k_next != k) {
+                widx += sk;
+                if (k_next < kend) ev = P.xys[widx + sk];
+            }
             k = k_next;
             edge_last = e_next;
         }

@@ -235,6 +235,7 @@ int64_t rge_build(const double* xi, const double* yi, int64_t nxi, int64_t nyi,
         const int64_t nxs = sweep_input ? nxi : nxo, nys = sweep_input ? nyi : nyo;
         P.xs = sweep_input ? xi : xo;
         P.ys = sweep_input ? yi : yo;
+        P.xys = sweep_input ? Gi.xy : Go.xy;
         if (axis == 0) { P.sr = 1; P.sk = nys; P.nrows = nys; P.nk = nxs; }
         else { P.sr = nys; P.sk = 1; P.nrows = nxs; P.nk = nys; }
         P.ncy_sweep = nys - 1;
