@@ -475,7 +475,7 @@ __global__ void k_row_start(const Build* __restrict__ builds,
         w = wave_sum(w);
         if (w != 0) {
             int64_t c = 0;
-            if (lane == 0) c = index_secant(s.x, s.y, s.nx, s.ny, px, py);
+            if (lane == 0) c = index_secant_xy(s.xy, s.nx, s.ny, px, py);
             c = __shfl(c, 0, 64);
             if (c == RG_SEARCH_BRUTE) c = wave_brute(s.x, s.y, s.nx, s.ny, px, py, lane);
             if (c >= 0) {

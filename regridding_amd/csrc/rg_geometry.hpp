@@ -217,4 +217,94 @@ RG_HD int64_t index_secant(const double* __restrict__ x,
     return RG_SEARCH_BRUTE;
 }
 
+// The cell search over an interleaved (x, y) vertex array: the same
+// arithmetic as bilinear / cell_contains / index_local / index_secant on
+// the separate arrays, with one 16-byte load per vertex instead of two.
+RG_HD double2 bilinear_xy(const double2* __restrict__ a, int64_t nx, int64_t ny, double x,
+                          double y)
+{
+    int64_t x0 = floor_i64(x), y0 = floor_i64(y);
+    if (x0 < 0) x0 = 0;
+    else if (x0 > nx - 2) x0 = nx - 2;
+    if (y0 < 0) y0 = 0;
+    else if (y0 > ny - 2) y0 = ny - 2;
+    const double2 a00 = a[x0 * ny + y0];
+    const double2 a01 = a[x0 * ny + y0 + 1];
+    const double2 a10 = a[(x0 + 1) * ny + y0];
+    const double2 a11 = a[(x0 + 1) * ny + y0 + 1];
+    double dx = x - (double)x0;
+    double dy = y - (double)y0;
+    double w00 = (1 - dx) * (1 - dy);
+    double w01 = (1 - dx) * dy;
+    double w10 = dx * (1 - dy);
+    double w11 = dx * dy;
+    double2 r;
+    r.x = (a00.x * w00) + (a01.x * w01) + (a10.x * w10) + (a11.x * w11);
+    r.y = (a00.y * w00) + (a01.y * w01) + (a10.y * w10) + (a11.y * w11);
+    return r;
+}
+
+RG_HD bool cell_contains_xy(const double2* __restrict__ xy, int64_t ny, int64_t i, int64_t j,
+                            double px, double py)
+{
+    int64_t a = i * ny + j, b = (i + 1) * ny + j;
+    const double2 v0 = xy[a], v1 = xy[b], v2 = xy[b + 1], v3 = xy[a + 1];
+    double w = 0;
+    w += winding_term(px, py, v3.x, v3.y, v0.x, v0.y);
+    w += winding_term(px, py, v0.x, v0.y, v1.x, v1.y);
+    w += winding_term(px, py, v1.x, v1.y, v2.x, v2.y);
+    w += winding_term(px, py, v2.x, v2.y, v3.x, v3.y);
+    return w != 0;
+}
+
+RG_HD int64_t index_local_xy(const double2* __restrict__ xy, int64_t nx, int64_t ny,
+                             int64_t i0, int64_t j0, double px, double py)
+{
+    int64_t ncx = nx - 1, ncy = ny - 1;
+    int64_t ia = i0 - 1 > 0 ? i0 - 1 : 0, ib = i0 + 2 < ncx ? i0 + 2 : ncx;
+    int64_t ja = j0 - 1 > 0 ? j0 - 1 : 0, jb = j0 + 2 < ncy ? j0 + 2 : ncy;
+    for (int64_t ii = ia; ii < ib; ii++)
+        for (int64_t jj = ja; jj < jb; jj++)
+            if (cell_contains_xy(xy, ny, ii, jj, px, py)) return ii * ncy + jj;
+    return -1;
+}
+
+RG_HD int64_t index_secant_xy(const double2* __restrict__ xy, int64_t nx, int64_t ny,
+                              double px, double py)
+{
+    const double h = 1e-3;
+    int64_t ncx = nx - 1, ncy = ny - 1;
+    double i = (double)nx / 2;
+    double j = (double)ny / 2;
+    for (int it = 0; it < 100; it++) {
+        const double2 v = bilinear_xy(xy, nx, ny, i, j);
+        const double xv = v.x, yv = v.y;
+        int64_t i0 = floor_i64(i), j0 = floor_i64(j);
+        double ex = xv - px;
+        double ey = yv - py;
+        if (i0 >= 0 && j0 >= 0 && i0 < ncx && j0 < ncy &&
+            cell_contains_xy(xy, ny, i0, j0, px, py)) {
+            int64_t c = index_local_xy(xy, nx, ny, i0, j0, px, py);
+            return c >= 0 ? c : RG_SEARCH_OUTSIDE;
+        }
+        if (__builtin_fabs(ex) < 1e-10 && __builtin_fabs(ey) < 1e-10) {
+            int64_t c = index_local_xy(xy, nx, ny, i0, j0, px, py);
+            return c >= 0 ? c : RG_SEARCH_OUTSIDE;
+        }
+        const double2 vi = bilinear_xy(xy, nx, ny, i + h, j);
+        const double2 vj = bilinear_xy(xy, nx, ny, i, j + h);
+        double dx_di = (vi.x - xv) / h;
+        double dx_dj = (vj.x - xv) / h;
+        double dy_di = (vi.y - yv) / h;
+        double dy_dj = (vj.y - yv) / h;
+        double det = dx_di * dy_dj - dx_dj * dy_di;
+        if (det == 0) return RG_SEARCH_BRUTE;
+        double di = (+dy_dj * ex - dx_dj * ey) / det;
+        double dj = (-dy_di * ex + dx_di * ey) / det;
+        i -= di;
+        j -= dj;
+    }
+    return RG_SEARCH_BRUTE;
+}
+
 }  // namespace rg

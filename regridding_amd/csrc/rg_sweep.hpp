@@ -463,7 +463,7 @@ RG_HD int32_t chunk_start_guess(const Pass& P, int64_t row, int64_t k)
     if (!(s.bbox[0] <= px && px <= s.bbox[2] && s.bbox[1] <= py &&
           py <= s.bbox[3]))
         return STATE_OUTSIDE;
-    int64_t c = index_secant(s.x, s.y, s.nx, s.ny, px, py);
+    int64_t c = index_secant_xy(s.xy, s.nx, s.ny, px, py);
     return c >= 0 ? (int32_t)c : STATE_OUTSIDE;
 }
 

@@ -483,4 +483,20 @@ double rge_winding(const double* bpx, const double* bpy, int64_t nbp, double px,
     return w;
 }
 
+// The cell search on separate x, y arrays and on the interleaved copy, for
+// n points: both results per point (they must be equal).
+void rge_secant_pair(const double* x, const double* y, int64_t nx, int64_t ny,
+                     const double* px, const double* py, int64_t n, int64_t* a, int64_t* b)
+{
+    std::vector<double2> xy(nx * ny);
+    for (int64_t q = 0; q < nx * ny; q++) {
+        xy[q].x = x[q];
+        xy[q].y = y[q];
+    }
+    for (int64_t q = 0; q < n; q++) {
+        a[q] = index_secant(x, y, nx, ny, px[q], py[q]);
+        b[q] = index_secant_xy(xy.data(), nx, ny, px[q], py[q]);
+    }
+}
+
 }  // extern "C"
