@@ -325,7 +325,8 @@ RG_HD int32_t walk_chunk(const Build& B, const Pass& P, int64_t row64,
     const int32_t limit = (int32_t)min(step_limit(s, k1 - k0), (int64_t)INT32_MAX - 1);
     int32_t k = (int32_t)k0;
     int32_t widx = row * (int32_t)P.sr + k * sk;  // sweep vertex (row, k)
-    double x1 = P.xs[widx], y1 = P.ys[widx];
+    const double2 sv0 = P.xys[widx];
+    double x1 = sv0.x, y1 = sv0.y;
     // the end of the current sweep edge, loaded when the walk reaches a
     // new edge (a crossing keeps it)
     double2 ev = make_double2(0, 0);
