@@ -220,6 +220,7 @@ struct GridDesc {
 struct PairCells {
     const double* xi;
     const double* yi;
+    const double2* xyi;  // the input grid's interleaved vertex copy
     int64_t nyi;
     const double* wi;
     int64_t cell_gbase, n_in;  // input cells: first (padded) global id, count
@@ -420,7 +421,7 @@ __global__ void k_pair_cells(const PairCells* __restrict__ pcs,
     double a = 1, w = 1;
     if (lc < P.n_in) {
         const int64_t ncy = P.nyi - 1;
-        a = cell_area(P.xi, P.yi, P.nyi, lc / ncy, lc % ncy);
+        a = cell_area_xy(P.xyi, P.nyi, lc / ncy, lc % ncy);
         if (P.wi) w = P.wi[lc];
     }
     vol[c] = a;
@@ -1757,6 +1758,7 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
         PairCells& P = pcs[k];
         P.xi = d.xi;
         P.yi = d.yi;
+        P.xyi = sg[2 * k].xy;
         P.nyi = d.nyi;
         P.wi = d.wi;
         any_wi = any_wi || d.wi;

@@ -156,6 +156,20 @@ RG_HD double cell_area(const double* __restrict__ x,
     return acc;
 }
 
+// cell_area on the interleaved vertex copy (the same operations in the
+// same order, one load per vertex).
+RG_HD double cell_area_xy(const double2* __restrict__ xy, int64_t ny, int64_t a, int64_t b)
+{
+    const int64_t v00 = a * ny + b, v10 = v00 + ny;
+    const double2 p00 = xy[v00], p10 = xy[v10], p01 = xy[v00 + 1], p11 = xy[v10 + 1];
+    double acc = 0.0;
+    acc -= area_tri(p00.y, p00.x, p10.y, p10.x);
+    acc += area_tri(p01.y, p01.x, p11.y, p11.x);
+    acc -= area_tri(p00.x, p00.y, p01.x, p01.y);
+    acc += area_tri(p10.x, p10.y, p11.x, p11.y);
+    return acc;
+}
+
 RG_HD int pass_of_chunk(const Build& B, int64_t g)
 {
     int p = 0;
