@@ -850,7 +850,7 @@ constexpr int RED_NET = 16;      // largest cell sorted in registers
 
 // One cell's terms: divided by the cell area (and weighted), sorted by key
 // in registers through a network of N, written back in place.
-template <int N>
+template <int N, int ST = 1>
 __device__ inline void sort_cell_in_registers(uint64_t* K, double* V, int32_t n, double vol,
                                               const double* wi, int64_t i)
 {
@@ -861,16 +861,16 @@ __device__ inline void sort_cell_in_registers(uint64_t* K, double* V, int32_t n,
         k[j] = ~(uint64_t)0;  // padding sorts last
         v[j] = 0;
         if (j < n) {
-            k[j] = K[j];
-            v[j] = weight_of(V[j], vol, wi, i);
+            k[j] = K[j * ST];
+            v[j] = weight_of(V[j * ST], vol, wi, i);
         }
     }
     sort_network<N, uint64_t>(k, v);
 #pragma unroll
     for (int j = 0; j < N; j++)
         if (j < n) {
-            K[j] = k[j];
-            V[j] = v[j];
+            K[j * ST] = k[j];
+            V[j * ST] = v[j];
         }
 }
 
@@ -907,19 +907,19 @@ constexpr int PUT_R = 8;
 
 __device__ inline void put_reduced_cell(KV* __restrict__ dst, const uint64_t* K,
                                         const double* V, int64_t m,
-                                        int32_t* __restrict__ hist_out)
+                                        int32_t* __restrict__ hist_out, int64_t st = 1)
 {
     for (int64_t t0 = 0; t0 < m; t0 += PUT_R) {
         uint32_t rk[PUT_R];
 #pragma unroll
         for (int u = 0; u < PUT_R; u++)
-            if (t0 + u < m) rk[u] = (uint32_t)atomicAdd(&hist_out[(int64_t)K[t0 + u]], 1);
+            if (t0 + u < m) rk[u] = (uint32_t)atomicAdd(&hist_out[(int64_t)K[(t0 + u) * st]], 1);
 #pragma unroll
         for (int u = 0; u < PUT_R; u++)
             if (t0 + u < m) {
                 KV r;
-                r.key = K[t0 + u] | ((uint64_t)rk[u] << 32);
-                r.val = V[t0 + u];
+                r.key = K[(t0 + u) * st] | ((uint64_t)rk[u] << 32);
+                r.val = V[(t0 + u) * st];
                 dst[t0 + u] = r;
             }
     }
@@ -1012,11 +1012,43 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
     const int32_t s = wave_exclusive(n, lane);
     cell_off[i] = base + s;
     const double vol = vol_in[i];
+    uint64_t* K = sk[wid];
+    double* V = sv[wid];
+    int32_t nmax = n;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o, 64));
+    if (nmax <= RED_NET) {
+        // every cell fits RED_NET entries: entry j of cell c sits at
+        // j * 64 + c, so the lanes (one per cell) touch consecutive LDS
+        // words at every step of the sort and the reduction
+        wave_sync_lds();
+        C[lane] = 0;
+        wave_sync_lds();
+#pragma unroll
+        for (int u = 0; u < RED_CAP / 64; u++)
+            if (cc[u] >= 0) {
+                const int32_t d = atomicAdd(&C[cc[u]], 1);
+                K[d * 64 + cc[u]] = rr[u].key;
+                V[d * 64 + cc[u]] = rr[u].val;
+            }
+        wave_sync_lds();
+        if (nmax <= RED_NET / 2)
+            sort_cell_in_registers<RED_NET / 2, 64>(K + lane, V + lane, n, vol, wi, i);
+        else
+            sort_cell_in_registers<RED_NET, 64>(K + lane, V + lane, n, vol, wi, i);
+        bool lr1 = false;  // (a run of at most RED_NET terms is never long)
+        const int64_t m1 = bucket_reduce_sorted(K + lane, V + lane, n, &lr1, 64);
+        const int64_t dst1 = wave_exclusive((int32_t)m1, lane);
+        int32_t* hist_out =
+            hist_out_all + pcs[__builtin_amdgcn_readfirstlane(pair_of_group[grp])].out_gbase;
+        put_reduced_cell(red + base + dst1, K + lane, V + lane, m1, hist_out, 64);
+        nnz_row[i] = (int32_t)m1;
+        if (lane == 0) group_mode[grp] = 0;
+        return;
+    }
     wave_sync_lds();
     C[lane] = s;  // placement cursors
     wave_sync_lds();
-    uint64_t* K = sk[wid];
-    double* V = sv[wid];
 #pragma unroll
     for (int u = 0; u < RED_CAP / 64; u++)
         if (cc[u] >= 0) {
@@ -1027,9 +1059,6 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
     wave_sync_lds();
     // cells of up to RED_NET records sort in registers (a sorting network:
     // no LDS round trips); larger ones sort in LDS
-    int32_t nmax = n;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o, 64));
     if (nmax <= RED_NET / 2) {
         sort_cell_in_registers<RED_NET / 2>(K + s, V + s, n, vol, wi, i);
     } else if (n <= RED_NET) {

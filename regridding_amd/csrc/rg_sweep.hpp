@@ -735,15 +735,16 @@ RG_HD void sort_records(KV* r, int64_t n)
 // overlap in a polygon of a few edges; if one does, the bucket is left
 // sorted but not compacted, *long_run is set, and bucket_reduce_long (run by
 // a separate kernel) finishes it.
+// (entry j of the bucket at keys[j * st], vals[j * st])
 RG_HD int64_t bucket_reduce_sorted(uint64_t* keys, double* vals, int64_t n,
-                                   bool* long_run)
+                                   bool* long_run, int64_t st = 1)
 {
     int64_t m = 0, p = 0;
     bool lr = false;
     while (p < n) {
-        uint64_t out = keys[p] >> 32;
+        uint64_t out = keys[p * st] >> 32;
         int64_t q = p + 1;
-        while (q < n && (keys[q] >> 32) == out) q++;
+        while (q < n && (keys[q * st] >> 32) == out) q++;
         if (q - p - 1 > 128) lr = true;
         m++;
         p = q;
@@ -753,12 +754,13 @@ RG_HD int64_t bucket_reduce_sorted(uint64_t* keys, double* vals, int64_t n,
     m = 0;
     p = 0;
     while (p < n) {
-        uint64_t out = keys[p] >> 32;
+        uint64_t out = keys[p * st] >> 32;
         int64_t q = p + 1;
-        while (q < n && (keys[q] >> 32) == out) q++;
-        double v = q - p == 1 ? vals[p] : vals[p] + pairwise_leaf(vals + p + 1, q - p - 1);
-        keys[m] = out;
-        vals[m] = v;
+        while (q < n && (keys[q * st] >> 32) == out) q++;
+        double v = q - p == 1 ? vals[p * st]
+                              : vals[p * st] + pairwise_leaf(vals + (p + 1) * st, q - p - 1, st);
+        keys[m * st] = out;
+        vals[m * st] = v;
         m++;
         p = q;
     }
