@@ -1049,24 +1049,37 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
     wave_sync_lds();
     C[lane] = s;  // placement cursors
     wave_sync_lds();
+    // the keys cell-major in LDS; then every record's place in its cell is
+    // the number of smaller keys of its cell (the keys are unique: they hold
+    // the emission ordinal), and the records are written there, weighted
 #pragma unroll
     for (int u = 0; u < RED_CAP / 64; u++)
-        if (cc[u] >= 0) {
-            const int32_t d = atomicAdd(&C[cc[u]], 1);
-            K[d] = rr[u].key;
-            V[d] = rr[u].val;
-        }
+        if (cc[u] >= 0) K[atomicAdd(&C[cc[u]], 1)] = rr[u].key;
     wave_sync_lds();
-    // cells of up to RED_NET records sort in registers (a sorting network:
-    // no LDS round trips); larger ones sort in LDS
-    if (nmax <= RED_NET / 2) {
-        sort_cell_in_registers<RED_NET / 2>(K + s, V + s, n, vol, wi, i);
-    } else if (n <= RED_NET) {
-        sort_cell_in_registers<RED_NET>(K + s, V + s, n, vol, wi, i);
-    } else {
-        for (int32_t q = 0; q < n; q++) V[s + q] = weight_of(V[s + q], vol, wi, i);
-        sort_pairs(K + s, V + s, (int64_t)n);
+    int32_t pos[RED_CAP / 64];
+#pragma unroll
+    for (int u = 0; u < RED_CAP / 64; u++) {
+        const int32_t ce = cc[u] >= 0 ? cc[u] : lane;
+        const int32_t cs = __shfl(s, ce, 64), cn = __shfl(n, ce, 64);
+        pos[u] = -1;
+        if (cc[u] >= 0) {
+            const uint64_t k = rr[u].key;
+            int32_t r = 0;
+            for (int32_t j = cs; j < cs + cn; j++) r += K[j] < k ? 1 : 0;
+            pos[u] = cs + r;
+        }
     }
+    wave_sync_lds();
+#pragma unroll
+    for (int u = 0; u < RED_CAP / 64; u++) {
+        const int32_t ce = cc[u] >= 0 ? cc[u] : lane;
+        const double cv = __shfl(vol, ce, 64);
+        if (pos[u] >= 0) {
+            K[pos[u]] = rr[u].key;
+            V[pos[u]] = weight_of(rr[u].val, cv, wi, grp * 64 + ce);
+        }
+    }
+    wave_sync_lds();
     bool lr = false;
     int64_t m = bucket_reduce_sorted(K + s, V + s, n, &lr);
     if (lr) m = n;  // left sorted, uncompacted, for k_bucket_long
