@@ -84,7 +84,7 @@ def main() -> int:
     if args.gpus != 1:
         print(
             "bench.py measures one GPU: the build and apply of a grid pair run on a "
-            "single MI355X (multi-GPU sharding is described in DESIGN.md, not run)",
+            "single MI355X (independent pairs would shard across GPUs; that is not run here)",
             file=sys.stderr,
         )
         return 2

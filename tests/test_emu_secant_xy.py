@@ -3,7 +3,7 @@ The build's cell search reads the interleaved (x, y) vertex copy
 (index_secant_xy) with the arithmetic of the reference's secant search on
 the separate arrays (index_secant, pinned against the oracle elsewhere): on
 the CPU (tests/emu) both return the same cell, the same "outside" and the
-same "exhaustive search" verdict for points inside, outside and on the
+same "exhaustive search" answer for points inside, outside and on the
 vertices of warped grids.
 """
 
