@@ -742,14 +742,25 @@ __global__ __launch_bounds__(WALK_MAX_BLOCK) void k_scatter(const Build* __restr
         H.cnt[h] = 0;
     }
     wave_sync_lds();
+    // the next round's triples are requested while this round is placed
+    Triple en[SCAT_R];
+#pragma unroll
+    for (int k = 0; k < SCAT_R; k++)
+        if (k < n) en[k] = k < cap ? stage[stage_index(t, k, cap)] : ov[k - cap];
     for (int32_t q0 = 0; q0 < mx; q0 += SCAT_R) {
         Triple e[SCAT_R];
+#pragma unroll
+        for (int k = 0; k < SCAT_R; k++) e[k] = en[k];
+#pragma unroll
+        for (int k = 0; k < SCAT_R; k++) {
+            const int32_t q = q0 + SCAT_R + k;
+            if (q < n) en[k] = q < cap ? stage[stage_index(t, q, cap)] : ov[q - cap];
+        }
         int32_t slot[SCAT_R], local[SCAT_R];
 #pragma unroll
         for (int k = 0; k < SCAT_R; k++) {
             slot[k] = -2;  // no triple
             if (q0 + k < n) {
-                e[k] = q0 + k < cap ? stage[stage_index(t, q0 + k, cap)] : ov[q0 + k - cap];
                 const int32_t b = e[k].cin >> 6;
                 int32_t h = b & (SCAT_HT - 1);
                 slot[k] = -1;  // table full: claim alone
