@@ -1358,25 +1358,42 @@ __global__ void k_csr_groups(const PairCells* __restrict__ pcs,
     const int64_t my_src = group_mode[grp] == 0 ? cell_off[i0]
                            : i < i1             ? cell_off[i] - (row_off[i] - d0)
                                                 : 0;
-    for (int64_t q0 = 0; q0 < cnt; q0 += 64) {  // uniform trip count: shuffles below
-        const int64_t q = q0 + lane;
-        // row of entry q = the last lane whose first entry is at or before q
-        int32_t row = 0;
+    // CSR_U entries per lane at a time: their records, then their output
+    // rows' offsets, are all requested before any is used
+    constexpr int CSR_U = 4;
+    for (int64_t q00 = 0; q00 < cnt; q00 += 64 * CSR_U) {  // uniform trip count: shuffles below
+        KV r[CSR_U];
+        int32_t rw[CSR_U];
 #pragma unroll
-        for (int b = 32; b > 0; b >>= 1) {
-            const int64_t dd = __shfl(my_d, (row + b) & 63, 64);
-            if (row + b < 64 && dd <= q) row += b;
+        for (int u = 0; u < CSR_U; u++) {
+            const int64_t q = q00 + 64 * u + lane;
+            // row of entry q = the last lane whose first entry is at or before q
+            int32_t row = 0;
+#pragma unroll
+            for (int b = 32; b > 0; b >>= 1) {
+                const int64_t dd = __shfl(my_d, (row + b) & 63, 64);
+                if (row + b < 64 && dd <= q) row += b;
+            }
+            rw[u] = row;
+            const int64_t src = __shfl(my_src, row, 64) + q;
+            if (q < cnt) r[u] = rec[src];
         }
-        const int64_t src = __shfl(my_src, row, 64) + q;
-        if (q < cnt) {
-            const KV r = rec[src];
-            const int32_t o = (int32_t)(uint32_t)r.key;
-            col[d0 + q] = o;
-            out_val[d0 + q] = r.val;
-            const int64_t pos = toff[o] + (int64_t)(r.key >> 32);
-            if (pos < nnz_cap) {
-                t_col[pos] = (int32_t)(i0 + row - P.cell_gbase);
-                t_val[pos] = r.val;
+        int64_t tb[CSR_U];
+#pragma unroll
+        for (int u = 0; u < CSR_U; u++)
+            if (q00 + 64 * u + lane < cnt) tb[u] = toff[(int32_t)(uint32_t)r[u].key];
+#pragma unroll
+        for (int u = 0; u < CSR_U; u++) {
+            const int64_t q = q00 + 64 * u + lane;
+            if (q < cnt) {
+                const int32_t o = (int32_t)(uint32_t)r[u].key;
+                col[d0 + q] = o;
+                out_val[d0 + q] = r[u].val;
+                const int64_t pos = tb[u] + (int64_t)(r[u].key >> 32);
+                if (pos < nnz_cap) {
+                    t_col[pos] = (int32_t)(i0 + rw[u] - P.cell_gbase);
+                    t_val[pos] = r[u].val;
+                }
             }
         }
     }
