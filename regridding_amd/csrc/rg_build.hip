@@ -1463,6 +1463,7 @@ __global__ __launch_bounds__(SORT_WAVES * 64) void k_rows_sort(
     if (total <= SORT_CAP) {
         int32_t* C = sc[wid];
         double* V = sv[wid];
+#pragma unroll 4
         for (int64_t q = lane; q < total; q += 64) {
             C[q] = col[base + q];
             V[q] = val[base + q];
