@@ -25,7 +25,7 @@ from ._lib import check, lib, rg_tuning
 
 __all__ = ["EDGES_PER_LANE", "BLOCKS", "signature", "tune", "cached"]
 
-EDGES_PER_LANE = (1, 2, 4, 8, 16, 32)
+EDGES_PER_LANE = (1, 2, 4, 8, 10, 12, 16, 32)
 BLOCKS = (64, 128, 256, 512)
 
 _lock = threading.Lock()
