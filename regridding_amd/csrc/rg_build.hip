@@ -406,6 +406,25 @@ __global__ void k_grid_super_boxes(const GridDesc* __restrict__ grids,
     bsup[sidx] = r;
 }
 
+// One thread per grid: the box of all its boundary edges (NaN ends ignored,
+// as in the block boxes) -- a sweep edge that misses it misses every block.
+__global__ void k_grid_boundary_box(const int64_t* __restrict__ sup_prefix, int ngrids,
+                                    const Box* __restrict__ bsup, Box* __restrict__ ball)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngrids) return;
+    Box r;
+    r.x0 = r.y0 = __builtin_huge_val();
+    r.x1 = r.y1 = -__builtin_huge_val();
+    for (int64_t q = sup_prefix[g]; q < sup_prefix[g + 1]; q++) {
+        r.x0 = fmin(r.x0, bsup[q].x0);
+        r.y0 = fmin(r.y0, bsup[q].y0);
+        r.x1 = fmax(r.x1, bsup[q].x1);
+        r.y1 = fmax(r.y1, bsup[q].y1);
+    }
+    ball[g] = r;
+}
+
 // Input cell areas (and the input weighting, 1 where a pair has none) over
 // the padded global cell space; padding cells get area 1 and weight 1 (they
 // receive no triples).
@@ -1782,7 +1801,7 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
     DevBuf<double2> xy_all;
     DevBuf<double> part, bbox_all, bpx, bpy, be;
     DevBuf<int32_t> bcell;
-    DevBuf<Box> bblk, bsup;
+    DevBuf<Box> bblk, bsup, ball;
     DevBuf<double2> pyr;
     RG_TRY(xy_all.alloc(v_tot, s));
     RG_TRY(part.alloc(4 * tile_tot, s));
@@ -1792,6 +1811,7 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
     RG_TRY(be.alloc(4 * e_tot, s));
     RG_TRY(bcell.alloc(e_tot, s));
     RG_TRY(bblk.alloc(blk_tot, s));
+    RG_TRY(ball.alloc(ng, s));
     RG_TRY(pyr.alloc(pyr_tot, s));
     RG_TRY(bsup.alloc(sup_tot, s));
     std::vector<StaticGrid> sg(ng);
@@ -1815,6 +1835,7 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
         S.pyr = pyr.p + G.pyr_base;
         S.npyr = pyr_pf[g + 1] - pyr_pf[g];
         S.bsup = bsup.p + G.sup_base;
+        S.ball = ball.p + g;
         S.nsup = sup_pf[g + 1] - sup_pf[g];
     }
 
@@ -1993,6 +2014,8 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
     RG_HIP(hipGetLastError());
     k_grid_super_boxes<<<grid_for(sup_tot, 256), 256, 0, s>>>(d_grids, d_sup, ng,
                                                               sup_tot, bblk.p, bsup.p);
+    RG_HIP(hipGetLastError());
+    k_grid_boundary_box<<<grid_for(ng, 64), 64, 0, s>>>(d_sup, ng, bsup.p, ball.p);
     RG_HIP(hipGetLastError());
     DevBuf<double> vol, wi_all;
     RG_TRY(vol.alloc(cell_tot, s));

@@ -55,6 +55,7 @@ struct StaticGrid {
     int64_t nblk;
     const Box* bsup;         // box per BND_SUPER blocks
     int64_t nsup;
+    const Box* ball;         // box of all boundary edges
     const double2* pyr;      // y range (min, max) of the vertices of each
     int64_t npyr;            // run of 64 boundary-polygon edges
 };
@@ -266,6 +267,7 @@ RG_HD bool outside_scan(const StaticGrid& s, double x1, double y1, double x2,
     const int32_t nsup = (int32_t)s.nsup, nblk = (int32_t)s.nblk, nbe = (int32_t)s.nbe;
     double t_min = __builtin_huge_val();
     int32_t best = -1;
+    if (!box_touch(*s.ball, xl, yl, xh, yh)) return false;
     for (int32_t su = 0; su < nsup; su++) {
         if (!box_touch(s.bsup[su], xl, yl, xh, yh)) continue;
         const int32_t b1 = min((su + 1) * BND_SUPER, nblk);

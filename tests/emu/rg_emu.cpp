@@ -92,7 +92,7 @@ struct Tables {
     std::vector<double2> xy;
     std::vector<double> bbox, bpx, bpy, be;
     std::vector<int32_t> bcell;
-    std::vector<Box> bblk, bsup;
+    std::vector<Box> bblk, bsup, ball;
 };
 
 void prepare(const double* x, const double* y, int64_t nx, int64_t ny,
@@ -148,12 +148,22 @@ void prepare(const double* x, const double* y, int64_t nx, int64_t ny,
         }
         T.bsup[s] = r;
     }
+    Box all{__builtin_huge_val(), __builtin_huge_val(), -__builtin_huge_val(),
+            -__builtin_huge_val()};
+    for (const Box& r : T.bsup) {
+        all.x0 = fmin(all.x0, r.x0);
+        all.y0 = fmin(all.y0, r.y0);
+        all.x1 = fmax(all.x1, r.x1);
+        all.y1 = fmax(all.y1, r.y1);
+    }
+    T.ball.assign(1, all);
     G.x = x; G.y = y; G.xy = T.xy.data(); G.nx = nx; G.ny = ny;
     G.bbox = T.bbox.data();
     G.bpx = T.bpx.data(); G.bpy = T.bpy.data(); G.nbp = nbe;
     G.be = T.be.data(); G.bcell = T.bcell.data(); G.nbe = nbe;
     G.bblk = T.bblk.data(); G.nblk = nblk;
     G.bsup = T.bsup.data(); G.nsup = nsup;
+    G.ball = T.ball.data();
 }
 
 // k_row_start, one wavefront serialised: winding terms summed in lane order
