@@ -36,7 +36,10 @@
  *                          answers; closed form (rectilinear overlap); brute
  *                          force polygon clipping; conservation invariants
  *   reduceat order         numpy.add.reduceat itself, bit for bit
- *   apply                  dense matrix product; regrid == weights + apply
+ *   apply                  numpy.add.at on shuffled triples with repeated
+ *                          pairs and mixed magnitudes, bit for bit (the
+ *                          sequential storage-order sum); dense matrix
+ *                          product; regrid == weights + apply
  *   find_indices 1-D       test_find_indices.py known answers; np.searchsorted
  */
 #include <math.h>

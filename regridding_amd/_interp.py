@@ -15,8 +15,11 @@ from ._lib import DeviceArray, check, lib
 __all__ = ["ndarray_linear_interpolation"]
 
 
-def _normalize_axis_tuple(axis, ndim):
-    return np.lib.array_utils.normalize_axis_tuple(axis, ndim=ndim)
+def _positions(axis, ndim: int) -> list[int]:
+    """Axis arguments as ascending non-negative positions."""
+    if axis is None:
+        return list(range(ndim))
+    return sorted(int(ax) % ndim for ax in np.lib.array_utils.normalize_axis_tuple(axis, ndim))
 
 
 def ndarray_linear_interpolation(
@@ -28,80 +31,64 @@ def ndarray_linear_interpolation(
     """
     Interpolate ``a`` at the fractional ``indices`` along ``axis`` (linear
     for one axis, bilinear for two), clamping to the first and last cell.
-    Same signature, broadcasting and errors as
-    ``regridding.ndarray_linear_interpolation``.
+    Same signature, broadcasting, result and exception types as
+    ``regridding.ndarray_linear_interpolation``
+    (``_interp_ndarray.py:11-165``):
+
+    * the k interpolated axes of ``a`` pair with the k index arrays in
+      ascending axis position (whatever order ``axis`` lists them in);
+    * the point axes of the indices (``axis_indices``, all by default) hold
+      the evaluation points; every other axis of either operand is
+      orthogonal, and the two operands must have the same number of
+      orthogonal axes, which broadcast against each other position by
+      position (``ValueError`` otherwise);
+    * the result has the shape of the broadcast indices.
+
+    The host only moves axes: the interpolated axes of ``a`` and the point
+    axes of the indices go last, the orthogonal shapes broadcast, and one
+    launch evaluates every point of every orthogonal element.
     """
     a = np.asarray(a)
-    shape_a = a.shape
-    ndim_a = a.ndim
-    indices_broadcasted = np.broadcast(*indices)
-    shape_indices = indices_broadcasted.shape
-    ndim_indices = indices_broadcasted.ndim
-
-    if axis is None:
-        axis = tuple(range(ndim_a))
-    axis = _normalize_axis_tuple(axis, ndim_a)
-    axis = tuple(~(~np.array(axis) % ndim_a))
-    if axis_indices is None:
-        axis_indices = tuple(range(ndim_indices))
-    axis_indices = _normalize_axis_tuple(axis_indices, ndim_indices)
-    axis_indices = tuple(~(~np.array(axis_indices) % ndim_indices))
-
-    if len(indices) != len(axis):
+    indices = tuple(np.asarray(ind) for ind in indices)
+    ndim_i = np.broadcast(*indices).ndim
+    ax_a = _positions(axis, a.ndim)
+    ax_i = _positions(axis_indices, ndim_i)
+    k = len(ax_a)
+    if len(indices) != k:
         raise ValueError(
-            f"The number of indices, {len(indices)}, must match the number of elements "
-            f"in axis, {len(axis)}"
+            f"ndarray_linear_interpolation: {len(indices)} index arrays were given for "
+            f"{k} interpolated axes"
         )
-
-    axis_orthogonal_a = tuple(ax for ax in range(-ndim_a, 0) if ax not in axis)
-    axis_orthogonal_indices = tuple(ax for ax in range(-ndim_indices, 0) if ax not in axis_indices)
-    shape_orthogonal_a = tuple(shape_a[ax] for ax in axis_orthogonal_a)
-    shape_orthogonal_indices = tuple(shape_indices[ax] for ax in axis_orthogonal_indices)
-    shape_orthogonal = np.broadcast_shapes(shape_orthogonal_a, shape_orthogonal_indices)
-
-    ndim_broadcasted_a = len(shape_orthogonal) + len(axis)
-    ndim_broadcasted_indices = len(shape_orthogonal) + len(axis_indices)
-    shape_broadcasted_a = tuple(
-        shape_a[ax] if ax in axis else shape_orthogonal[axis_orthogonal_a.index(ax)]
-        for ax in range(-ndim_broadcasted_a, 0)
-    )
-    shape_broadcasted_indices = tuple(
-        shape_indices[ax] if ax in axis_indices
-        else shape_orthogonal[axis_orthogonal_indices.index(ax)]
-        for ax in range(-ndim_broadcasted_indices, 0)
-    )
-    a = np.broadcast_to(a, shape_broadcasted_a)
-    indices = tuple(np.broadcast_to(ind, shape_broadcasted_indices) for ind in indices)
-
-    if len(axis) not in (1, 2):
-        raise NotImplementedError
-
-    # orthogonal axes first (in the reference's ndindex order), then the
-    # interpolated axes of `a` in ascending order -- the order in which the
-    # reference's a[index_a] presents them to the evaluator -- and the point
-    # axes of the indices
-    orth_a = tuple(ax for ax in range(-ndim_broadcasted_a, 0) if ax not in axis)
-    orth_i = tuple(ax for ax in range(-ndim_broadcasted_indices, 0) if ax not in axis_indices)
-    axis_a = tuple(ax for ax in range(-ndim_broadcasted_a, 0) if ax in axis)
-    a_m = np.moveaxis(a, orth_a + axis_a, tuple(range(ndim_broadcasted_a)))
-    D = int(np.prod(shape_orthogonal))
-    a_m = np.ascontiguousarray(
-        a_m.reshape((D,) + tuple(shape_broadcasted_a[ax] for ax in axis_a)), dtype=np.float64)
-    pts_shape = tuple(shape_broadcasted_indices[ax] for ax in axis_indices)
+    n_orth = a.ndim - k
+    if ndim_i - len(ax_i) != n_orth:
+        raise ValueError(
+            f"ndarray_linear_interpolation: `a` has {n_orth} orthogonal axes but the indices "
+            f"have {ndim_i - len(ax_i)}"
+        )
+    last_a = tuple(range(-k, 0))
+    last_i = tuple(range(-len(ax_i), 0))
+    a_m = np.moveaxis(a, ax_a, last_a)
+    p_m = [np.moveaxis(np.broadcast_to(ind, np.broadcast(*indices).shape), ax_i, last_i)
+           for ind in indices]
+    grid_shape = a_m.shape[n_orth:]
+    pts_shape = p_m[0].shape[n_orth:]
+    shape_orth = np.broadcast_shapes(a_m.shape[:n_orth], p_m[0].shape[:n_orth])
+    if k not in (1, 2):
+        raise NotImplementedError("interpolation along more than two axes")
+    D = int(np.prod(shape_orth))
     m = int(np.prod(pts_shape))
-    ind_m = [
-        np.ascontiguousarray(
-            np.moveaxis(ind, orth_i + axis_indices, tuple(range(ndim_broadcasted_indices)))
-            .reshape(D, m), dtype=np.float64)
-        for ind in indices
-    ]
+    a_flat = np.ascontiguousarray(
+        np.broadcast_to(a_m, shape_orth + grid_shape).reshape((D,) + grid_shape),
+        dtype=np.float64)
+    p_flat = [np.ascontiguousarray(np.broadcast_to(p, shape_orth + pts_shape).reshape(D, m),
+                                   dtype=np.float64) for p in p_m]
     out = DeviceArray.empty((D, m), np.float64)
-    da = DeviceArray.from_host(a_m)
-    dx = [DeviceArray.from_host(v) for v in ind_m]
-    if len(axis) == 1:
-        check(lib().rg_interp_linear_1d(da.ptr, a_m.shape[1], dx[0].ptr, m, D, out.ptr, 0))
+    da = DeviceArray.from_host(a_flat)
+    dp = [DeviceArray.from_host(p) for p in p_flat]
+    if k == 1:
+        check(lib().rg_interp_linear_1d(da.ptr, grid_shape[0], dp[0].ptr, m, D, out.ptr, 0))
     else:
-        check(lib().rg_interp_linear_2d(da.ptr, a_m.shape[1], a_m.shape[2], dx[0].ptr,
-                                        dx[1].ptr, m, D, out.ptr, 0))
-    res = out.to_host().reshape(tuple(shape_orthogonal) + pts_shape)
-    return np.moveaxis(res, tuple(range(ndim_broadcasted_indices)), orth_i + axis_indices)
+        check(lib().rg_interp_linear_2d(da.ptr, grid_shape[0], grid_shape[1], dp[0].ptr,
+                                        dp[1].ptr, m, D, out.ptr, 0))
+    res = out.to_host().reshape(shape_orth + pts_shape)
+    return np.moveaxis(res, last_i, ax_i)

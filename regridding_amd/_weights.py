@@ -117,8 +117,9 @@ def weights(
     orthogonal axes whose elements are ``(indices_input, indices_output,
     values)`` flat arrays sorted by ``(input, output)`` with unique pairs,
     plus the input and output value shapes.  The builds run on the GPU; the
-    returned arrays stay resident there for :func:`regrid_from_weights`
-    (they are returned read-only for that reason).
+    weights stay resident there for :func:`regrid_from_weights`, which uses
+    the resident copy while the returned (writable) arrays still hold the
+    bytes they were returned with, and imports them again once edited.
     """
     unit_weights = getattr(weights_input, "unit", None)
     if unit_weights is not None:

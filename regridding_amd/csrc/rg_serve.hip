@@ -51,6 +51,7 @@ struct Req {
     uint64_t ticket = 0;
     int64_t footprint = 0;
     bool done = false;
+    bool failed_alone = false;  // rejected by itself; the group went on
     int status = RG_OK;
     std::string error;
     int64_t done_ns = 0;
@@ -313,6 +314,21 @@ void rg_server::run_group(hipStream_t s, std::vector<Req*>& group,
                     if (build_reqs[k] == q) w = built[k];
                 uncached.push_back(w);
             }
+            // the caller's buffers are sized from the request's shapes: a
+            // cached key whose weights have other sizes (wrong shapes, or a
+            // key collision) fails this request alone
+            if (w->n_in != cells_in(r) || w->n_out != cells_out(r)) {
+                q->failed_alone = true;
+                q->status = fail(RG_ERR_ARGUMENT,
+                                 "the cached weights of this key map " +
+                                     std::to_string(w->n_in) + " -> " +
+                                     std::to_string(w->n_out) +
+                                     " cells, the request's shapes " +
+                                     std::to_string(cells_in(r)) + " -> " +
+                                     std::to_string(cells_out(r)));
+                q->error = rg_last_error();
+                continue;
+            }
             const int64_t es = r.dtype == RG_F32 ? 4 : 8;
             const int64_t nsrc = r.transpose ? w->n_out : w->n_in;
             const int64_t ndst = r.transpose ? w->n_in : w->n_out;
@@ -384,10 +400,12 @@ void rg_server::run_group(hipStream_t s, std::vector<Req*>& group,
             }
             admitted -= q->footprint;
             q->done = true;
-            q->status = status;
-            q->error = error;
+            if (!q->failed_alone) {
+                q->status = status;
+                q->error = error;
+            }
             q->done_ns = t;
-            if (status == RG_OK) st.completed++;
+            if (q->status == RG_OK) st.completed++;
             else st.failed++;
         }
         evict_locked();

@@ -79,12 +79,16 @@ def grid_key(grid_input, grid_output, weights_input=None) -> int:
 
 
 class Ticket:
-    __slots__ = ("id", "result", "_keep", "arrival_ns")
+    """A submitted request: its id, its result buffer and its arrival time.
+    The host buffers the engine reads and writes are held by the server until
+    the request is waited for or the server closes, so dropping a ticket
+    early is safe."""
 
-    def __init__(self, tid, result, keep, arrival_ns):
+    __slots__ = ("id", "result", "arrival_ns")
+
+    def __init__(self, tid, result, arrival_ns):
         self.id = tid
         self.result = result
-        self._keep = keep
         self.arrival_ns = arrival_ns
 
 
@@ -98,6 +102,8 @@ class Server:
         cfg = rg_serve_config(workers, max_group, reserve_bytes, cache_bytes,
                               rg_tuning(edges_per_lane, block, 0, 0))
         self._h = ctypes.c_void_p()
+        # host buffers of the requests not yet waited for, by ticket id
+        self._pending: dict[int, list] = {}
         check(L.rg_serve_create(ctypes.byref(cfg), ctypes.byref(self._h)))
 
     def submit(self, values: np.ndarray, grid_input=None, grid_output=None, key: int = 0,
@@ -146,14 +152,15 @@ class Server:
         r.arrival_ns = int(arrival_ns)
         t = ctypes.c_uint64()
         check(L.rg_serve_submit(self._h, ctypes.byref(r), ctypes.byref(t)))
-        return Ticket(t.value, result, keep, r.arrival_ns)
+        self._pending[t.value] = keep
+        return Ticket(t.value, result, r.arrival_ns)
 
     def wait(self, ticket: Ticket) -> tuple[np.ndarray, int]:
         """The request's result and its completion time (rg_clock_ns)."""
         L = _bind()
         done = ctypes.c_int64()
         status = L.rg_serve_wait(self._h, ticket.id, ctypes.byref(done))
-        ticket._keep = None
+        self._pending.pop(ticket.id, None)
         if status:
             raise RGError(status, L.rg_last_error().decode(errors="replace"))
         return ticket.result, int(done.value)
@@ -164,9 +171,12 @@ class Server:
         return {name: getattr(s, name) for name, _ in rg_serve_stats._fields_}
 
     def close(self) -> None:
+        """Finish every submitted request, stop the workers, free the cache;
+        then release the buffers of requests nobody waited for."""
         if self._h and self._h.value:
             check(_bind().rg_serve_destroy(self._h))
             self._h = None
+        self._pending.clear()
 
     def __enter__(self):
         return self

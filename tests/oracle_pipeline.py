@@ -137,3 +137,40 @@ def weights_multilinear(coordinates_input, coordinates_output, axis_input=None,
             raise ValueError("outside")
         flat[d] = oracle.coalesce(ii, io, w)
     return flat.reshape(shape_orth), tuple(shape_in), tuple(shape_out)
+
+
+def transpose_weights_conservative(weights, coordinates_input, coordinates_output,
+                                   axis_input=None, axis_output=None, weights_input=None):
+    """``transpose_weights_conservative`` (``_weights_transposed.py:53-259``)
+    with the oracle's cell areas (``rgo_grid_volume``) on the unperturbed
+    grids: element by element, ``w / wi[i]**2 * area_in[i] / area_out[o]``
+    in numpy's operation order, indices swapped."""
+    weights_array, shape_values_in, shape_values_out = weights
+    (ci, co, axis_input, axis_output, shape_in, shape_out, shape_orth) = (
+        _util.normalize_coordinates(coordinates_input, coordinates_output, axis_input,
+                                    axis_output, False, _util.SEED_DEFAULT)
+    )
+
+    def areas(grid, axes, index):
+        sl = _util.orthogonal_slice(index, axes)
+        if len(grid) == 2:
+            return oracle.grid_volume(grid[0][sl], grid[1][sl]).ravel()
+        x = np.asarray(grid[0][sl], dtype=np.float64)
+        return x[1:] - x[:-1]
+
+    wi_full = None
+    if weights_input is not None:
+        sv_in = list(shape_in)
+        for ax in axis_input:
+            sv_in[ax] -= 1
+        wi_full = np.broadcast_to(weights_input, tuple(sv_in))
+    result = np.empty(weights_array.shape, dtype=object)
+    for index in np.ndindex(*shape_orth):
+        ii, io, v = weights_array[index]
+        v = np.array(v, dtype=np.float64)
+        if wi_full is not None:
+            wi = np.asarray(wi_full[_util.orthogonal_slice(index, axis_input)]).ravel()
+            v = v / np.square(wi[ii])
+        v = v * areas(ci, axis_input, index)[ii] / areas(co, axis_output, index)[io]
+        result[index] = (io, ii, v)
+    return result, shape_values_out, shape_values_in

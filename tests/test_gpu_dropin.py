@@ -206,3 +206,34 @@ def test_find_indices_2d_secant():
         for b in range(3):
             e = oracle.index_of_point_secant((pts[0][a, b], pts[1][a, b]), g)
             assert (i[a, b], j[a, b]) == e
+
+
+def test_returned_weights_are_writable_and_edits_apply():
+    """The reference returns fresh writable arrays (_weights_arrays.py:68-72);
+    an in-place edit of the values (or indices) must reach the next apply."""
+    w = rg.weights((xb, yb), (xo, yo), method="conservative")
+    ii, io, v = w[0][()]
+    assert ii.flags.writeable and io.flags.writeable and v.flags.writeable
+    before = rg.regrid_from_weights(*w, values_input=values_input)
+    v *= 2.0
+    after = rg.regrid_from_weights(*w, values_input=values_input)
+    wo = P.weights((xb, yb), (xo, yo))
+    oi, oo, ov = wo[0][()]
+    edited = np.empty((), dtype=object)
+    edited[()] = (oi, oo, ov * 2.0)
+    assert np.array_equal(after, P.regrid_from_weights(edited, wo[1], wo[2],
+                                                       values_input=values_input))
+    assert not np.array_equal(before, after)
+    # an index edit: output cell of the first entry moved
+    io[0] = (io[0] + 1) % int(np.prod(w[2]))
+    edited[()] = (oi, io.copy(), ov * 2.0)
+    got = rg.regrid_from_weights(*w, values_input=values_input)
+    assert np.array_equal(got, P.regrid_from_weights(edited, wo[1], wo[2],
+                                                     values_input=values_input))
+    # the transposed swap of the edited arrays applies the edited weights too
+    wt = rg.transpose_weights(w)
+    y = np.random.default_rng(3).random(w[2])
+    et = np.empty((), dtype=object)
+    et[()] = (io.copy(), oi, ov * 2.0)
+    assert np.array_equal(rg.regrid_from_weights(*wt, values_input=y),
+                          P.regrid_from_weights(et, wo[2], wo[1], values_input=y))

@@ -132,3 +132,44 @@ def test_failing_request_fails_alone():
             srv.wait(bad)
         good = srv.submit(x, ci, co)
         assert np.array_equal(srv.wait(good)[0], _reference(ci, co, x, False))
+
+
+def test_cached_key_with_wrong_shapes_fails_alone():
+    """A request whose shapes disagree with the cached weights of its key
+    fails with an argument error (its buffers are sized from its shapes);
+    the requests grouped with it complete."""
+    from regridding_amd._lib import RGError
+
+    pairs = _pairs(2, 53)
+    (ci, co), (cj, cq) = pairs
+    key = grid_key(ci, co)
+    with Server(workers=1, max_group=8) as srv:
+        x = np.ones((2, (ci[0].shape[0] - 1) * (ci[0].shape[1] - 1)))
+        first = srv.submit(x, ci, co, key=key)
+        ref = srv.wait(first)[0].copy()
+        # wrong shapes for the key: the other pair's shapes
+        bad_shapes = (cj[0].shape, cq[0].shape)
+        xb_ = np.ones((1, (cj[0].shape[0] - 1) * (cj[0].shape[1] - 1)))
+        bad = srv.submit(xb_, key=key, shapes=bad_shapes)
+        good = srv.submit(x, key=key, shapes=(ci[0].shape, co[0].shape))
+        with pytest.raises(RGError, match="cached weights of this key"):
+            srv.wait(bad)
+        assert np.array_equal(srv.wait(good)[0], ref)
+        st = srv.stats()
+    assert st["failed"] == 1
+
+
+def test_dropped_ticket_keeps_buffers_alive():
+    """A ticket dropped without wait(): the server holds the request's host
+    buffers until it has finished with them."""
+    import gc
+
+    pairs = _pairs(1, 59)
+    ci, co = pairs[0]
+    with Server(workers=1) as srv:
+        x = np.random.default_rng(0).random((3, (ci[0].shape[0] - 1) * (ci[0].shape[1] - 1)))
+        srv.submit(x.copy(), ci, co)  # ticket and values dropped at once
+        gc.collect()
+        tk = srv.submit(x, ci, co)
+        res = srv.wait(tk)[0]
+    assert np.array_equal(res, _reference(ci, co, x, False))
