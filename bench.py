@@ -213,6 +213,10 @@ def main() -> int:
 
     ms_apply64 = time_apply(L.rg_apply_f64, dx64, dy64, 0)
     ms_apply32 = time_apply(L.rg_apply_f32, dx32, dy32, 0)
+    y64_host = dy64.to_host()
+    y32_host = dy32.to_host()
+    f32_err = f32_max_rel_err(y32_host, y64_host)
+    log(f"FP32 apply: max relative error {f32_err:.3g} against the FP64 apply")
     ms_adjoint64 = time_apply(L.rg_apply_f64, dy64, dt64, 1)
     by64 = apply_bytes(n_in, n_out, nnz, B, 8)
     by32 = apply_bytes(n_in, n_out, nnz, B, 4)
@@ -302,7 +306,9 @@ def main() -> int:
         "build_fixups": st["n_fixups"],
         "build_restaged": st["n_restaged"],
         "apply_f64": {"ms": round(ms_apply64, 5), "GBps": round(gbps64, 1), "frac_hbm": round(gbps64 / HBM_PEAK_GBPS, 4)},
-        "apply_f32": {"ms": round(ms_apply32, 5), "GBps": round(gbps32, 1), "frac_hbm": round(gbps32 / HBM_PEAK_GBPS, 4)},
+        "apply_f32": {"ms": round(ms_apply32, 5), "GBps": round(gbps32, 1), "frac_hbm": round(gbps32 / HBM_PEAK_GBPS, 4),
+                      "max_rel_err": f32_err,
+                      "max_rel_err_against": "the FP64 apply (bit-identical to the FP64 oracle)"},
         "adjoint_f64": {"ms": round(ms_adjoint64, 5), "GBps": round(gbpst64, 1), "frac_hbm": round(gbpst64 / HBM_PEAK_GBPS, 4)},
         "wall_ms_per_step": round(wall_ms / args.steps, 4),
         "copy_GBps": round(copy_gbps, 1),
@@ -318,7 +324,7 @@ def main() -> int:
     if not args.no_inversion:
         result["inversion"] = bench_inversion(args)
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(ci, co, values, build())
+        result["cpu_baseline"] = cpu_baseline(ci, co, values, build(), y64_host, y32_host)
     print(json.dumps(result), flush=True)
     return 0
 
@@ -465,76 +471,77 @@ def bench_batched(args) -> dict:
     }
 
 
-def bench_serve(args) -> dict:
-    """The serving loop under a Poisson arrival stream: 60% of requests
-    reuse one of 64 grid pairs whose weights are cached on the device, 40%
-    bring a pair that is built on arrival (drawn from a pool of 128
-    pre-generated pairs and submitted uncached, so each one is a full
-    build); per-axis vertices log-uniform in [33, 1025]; 1-32 value arrays
-    per request, FP32 or FP64 50/50, drawn as slices of a pinned pool of
-    Poisson counts.  Latency is scheduled arrival -> result in host memory."""
-    import threading
-    from collections import deque
+class _ServeMix:
+    """The serving workload's request mix: 60% of requests reuse one of 64
+    grid pairs whose weights are cached on the device, 40% bring a pair
+    that is built on arrival (drawn from a pool of 128 pre-generated pairs
+    and submitted uncached, so each one is a full build); per-axis vertices
+    log-uniform in [33, 1025]; 1-32 value arrays per request, FP32 or FP64
+    50/50, drawn as slices of a pinned pool of Poisson counts."""
 
-    from regridding_amd import _util, workloads
-    from regridding_amd.serve import Server, clock_ns, grid_key, pinned_empty
+    def __init__(self, rng):
+        from regridding_amd import _util, workloads
+        from regridding_amd.serve import grid_key, pinned_empty
 
-    rng = np.random.default_rng(4)
-    t0 = time.perf_counter()
+        def make(n, seed):
+            out = []
+            for p in workloads.batched_pairs(n_pairs=n, seed=seed):
+                ci, co, *_ = _util.normalize_coordinates(p["grid_input"], p["grid_output"],
+                                                         perturb=True)
+                out.append((ci, co))
+            return out
 
-    def make(n, seed):
-        out = []
-        for p in workloads.batched_pairs(n_pairs=n, seed=seed):
-            ci, co, *_ = _util.normalize_coordinates(p["grid_input"], p["grid_output"], perturb=True)
-            out.append((ci, co))
-        return out
+        self.rng = rng
+        self.cached = make(64, 40)
+        self.fresh = make(128, 41)
+        self.keys = [grid_key(ci, co) for ci, co in self.cached]
+        self.pool_n = 40 << 20
+        self.pool64 = pinned_empty((self.pool_n,), np.float64)
+        self.pool64[:] = rng.poisson(20.0, self.pool_n)
+        self.pool32 = pinned_empty((self.pool_n,), np.float32)
+        self.pool32[:] = self.pool64
 
-    cached = make(64, 40)
-    fresh = make(128, 41)
-    keys = [grid_key(ci, co) for ci, co in cached]
-    pool_n = 40 << 20
-    pool64 = pinned_empty((pool_n,), np.float64)
-    pool64[:] = rng.poisson(20.0, pool_n)
-    pool32 = pinned_empty((pool_n,), np.float32)
-    pool32[:] = pool64
-    ring_bytes = 6 << 30
-    ring = pinned_empty((ring_bytes,), np.uint8)
-    log(f"serving setup {time.perf_counter() - t0:.1f} s")
-
+    @staticmethod
     def cells(g):
         return (g[0].shape[0] - 1) * (g[0].shape[1] - 1)
 
-    # arrivals and request shapes, fixed in advance
-    n_req = int(args.serve_rate * args.serve_seconds * 1.2) + 16
-    gaps = rng.exponential(1.0 / args.serve_rate, n_req)
-    arrive = np.cumsum(gaps)
-    arrive = arrive[arrive < args.serve_seconds]
-    plan = []
-    for _ in range(arrive.size):
+    def draw(self):
+        """(reuse, pair index, value stack) of the next request."""
+        rng = self.rng
         reuse = rng.random() < 0.6
         j = int(rng.integers(64 if reuse else 128))
         B = int(rng.integers(1, 33))
         f32 = bool(rng.random() < 0.5)
-        plan.append((reuse, j, B, f32))
+        ci, _ = self.cached[j] if reuse else self.fresh[j]
+        n_in = self.cells(ci)
+        pool = self.pool32 if f32 else self.pool64
+        off = int(rng.integers(0, self.pool_n - B * n_in + 1))
+        return reuse, j, pool[off: off + B * n_in].reshape(B, n_in)
 
-    srv = Server(workers=3, max_group=64)
-    # warm the cache with the 64 reusable pairs (not timed)
-    warm = []
-    for j, (ci, co) in enumerate(cached):
-        x = pool64[: cells(ci)].reshape(1, -1)
-        warm.append(srv.submit(x, ci, co, key=keys[j]))
-    for tk in warm:
-        srv.wait(tk)
 
+def _drive(srv, mix, ring, n_max, seconds, rate=None, window=None):
+    """Submit requests of `mix` to `srv` for `seconds`: at Poisson arrival
+    times of `rate` per second (open loop), or keeping `window` requests in
+    flight (closed loop).  Results land in slices of the pinned byte `ring`,
+    released in submission order.  Returns (latencies in ms, completions,
+    elapsed seconds from the first arrival to the last completion)."""
+    import threading
+    from collections import deque
+
+    from regridding_amd.serve import clock_ns
+
+    ring_bytes = ring.size
     lat = []
     inflight = deque()
     lock = threading.Condition()
     live = deque()  # result slices of the ring in submission order
-    state = {"done": False}
+    state = {"done": False, "error": None, "last_ns": 0}
+
+    def check_failed():
+        if state["error"] is not None:
+            raise RuntimeError("serving request failed") from state["error"]
 
     def ring_take(n):
-        # a contiguous slice of the pinned result ring; slices are released
-        # in submission order (the completer waits in that order)
         with lock:
             while True:
                 if not live:
@@ -554,11 +561,6 @@ def bench_serve(args) -> dict:
                 check_failed()
                 lock.wait()
 
-    def ring_give():
-        with lock:
-            live.popleft()
-            lock.notify_all()
-
     def completer():
         try:
             while True:
@@ -567,73 +569,209 @@ def bench_serve(args) -> dict:
                         lock.wait()
                     if not inflight and state["done"]:
                         return
-                    tk = inflight.popleft()
+                    tk = inflight[0]
                 _, done_ns = srv.wait(tk)
                 lat.append((done_ns - tk.arrival_ns) * 1e-6)
-                ring_give()
+                with lock:
+                    inflight.popleft()
+                    live.popleft()
+                    state["last_ns"] = max(state["last_ns"], done_ns)
+                    lock.notify_all()
         except BaseException as e:  # hand the failure to the submitting loop
             with lock:
                 state["error"] = e
                 lock.notify_all()
 
-    def check_failed():
-        if state.get("error") is not None:
-            raise RuntimeError("serving request failed") from state["error"]
-
     th = threading.Thread(target=completer)
     th.start()
     base = clock_ns() + 50_000_000
-    stats0 = srv.stats()
-    for (reuse, j, B, f32), t in zip(plan, arrive):
-        ci, co = cached[j] if reuse else fresh[j]
-        n_in, n_out = cells(ci), cells(co)
-        pool = pool32 if f32 else pool64
-        es = 4 if f32 else 8
-        off = int(rng.integers(0, pool_n - B * n_in + 1))
-        x = pool[off: off + B * n_in].reshape(B, n_in)
-        start = ring_take(B * n_out * es)
-        res = ring[start: start + B * n_out * es].view(np.float32 if f32 else np.float64).reshape(B, n_out)
-        at = base + int(t * 1e9)
-        while clock_ns() < at:
-            time.sleep(max(0.0, (at - clock_ns()) * 1e-9 - 2e-4))
-        if reuse:
-            tk = srv.submit(x, key=keys[j], shapes=(ci[0].shape, co[0].shape), result=res, arrival_ns=at)
-        else:
-            tk = srv.submit(x, ci, co, key=0, result=res, arrival_ns=at)
+    t_arr = 0.0
+    n = 0
+    try:
+        while n < n_max:
+            if rate is not None:
+                t_arr += float(mix.rng.exponential(1.0 / rate))
+                if t_arr >= seconds:
+                    break
+                at = base + int(t_arr * 1e9)
+            else:
+                if clock_ns() >= base + int(seconds * 1e9):
+                    break
+                at = 0
+            reuse, j, x = mix.draw()
+            ci, co = mix.cached[j] if reuse else mix.fresh[j]
+            n_out = mix.cells(co)
+            es = x.dtype.itemsize
+            nb = x.shape[0] * n_out * es
+            start = ring_take(nb)
+            res = ring[start: start + nb].view(x.dtype).reshape(x.shape[0], n_out)
+            if rate is not None:
+                while clock_ns() < at:
+                    time.sleep(max(0.0, (at - clock_ns()) * 1e-9 - 2e-4))
+            else:
+                with lock:
+                    while len(inflight) >= window:
+                        check_failed()
+                        lock.wait()
+                at = max(clock_ns(), base)
+                while clock_ns() < at:
+                    time.sleep(1e-4)
+            if reuse:
+                tk = srv.submit(x, key=mix.keys[j], shapes=(ci[0].shape, co[0].shape),
+                                result=res, arrival_ns=at)
+            else:
+                tk = srv.submit(x, ci, co, key=0, result=res, arrival_ns=at)
+            n += 1
+            with lock:
+                check_failed()
+                inflight.append(tk)
+                lock.notify_all()
+    finally:
         with lock:
-            check_failed()
-            inflight.append(tk)
+            state["done"] = True
             lock.notify_all()
-    with lock:
-        state["done"] = True
-        lock.notify_all()
-    th.join()
+        th.join()
     check_failed()
-    end_ns = clock_ns()
+    return np.asarray(lat), n, (state["last_ns"] - base) * 1e-9
+
+
+def _lat_stats(lat):
+    return {"p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "mean_ms": round(float(lat.mean()), 3)}
+
+
+def bench_serve(args) -> dict:
+    """The serving loop (continuous batching) under the serving workload:
+
+    * `offered`: Poisson arrivals at --serve-rate (2000/s) for --serve-seconds
+      (30 s): p50/p99 latency (scheduled arrival -> result in host memory) and
+      throughput;
+    * `capacity`: a closed loop keeping 256 requests in flight for 8 s: the
+      loop's maximum sustained request rate;
+    * `rate_sweep`: Poisson arrivals at fractions of that capacity, 4 s each:
+      the highest offered rate whose p99 stays within P99_BOUND_MS and whose
+      throughput keeps up with the arrivals;
+    * `plain`: the same request mix served one request per call (build a
+      fresh pair with rg_build_conservative_2d, apply with rg_apply_*,
+      synchronous copies; the 64 reusable pairs' weights kept): the
+      one-pair-per-call GPU path the served rate is compared with."""
+    from regridding_amd.serve import Server, pinned_empty
+
+    P99_BOUND_MS = 25.0
+    rng = np.random.default_rng(4)
+    t0 = time.perf_counter()
+    mix = _ServeMix(rng)
+    ring = pinned_empty((6 << 30,), np.uint8)
+    log(f"serving setup {time.perf_counter() - t0:.1f} s")
+
+    srv = Server(workers=3, max_group=64)
+    # warm the cache with the 64 reusable pairs (not timed)
+    warm = []
+    for j, (ci, co) in enumerate(mix.cached):
+        x = mix.pool64[: mix.cells(ci)].reshape(1, -1)
+        warm.append(srv.submit(x, ci, co, key=mix.keys[j]))
+    for tk in warm:
+        srv.wait(tk)
+
+    def leg(**kw):
+        s0 = srv.stats()
+        lat, n, wall = _drive(srv, mix, ring, **kw)
+        s1 = srv.stats()
+        d = {k: s1[k] - s0[k] for k in ("completed", "failed", "builds", "cache_hits", "groups")}
+        out = {"requests": int(n), **_lat_stats(lat),
+               "throughput_req_per_s": round(d["completed"] / wall, 1),
+               "builds": d["builds"], "cache_hits": d["cache_hits"], "groups": d["groups"],
+               "failed": d["failed"]}
+        return out
+
+    offered = leg(n_max=10**9, seconds=args.serve_seconds, rate=args.serve_rate)
+    offered["offered_req_per_s"] = round(offered["requests"] / args.serve_seconds, 1)
+    log(f"serving at {args.serve_rate:.0f}/s: {offered}")
+    cap = leg(n_max=10**9, seconds=8.0, window=256)
+    capacity = cap["throughput_req_per_s"]
+    log(f"serving capacity (256 in flight): {cap}")
+    sweep = []
+    for frac in (0.5, 0.7, 0.85, 1.0):
+        r = leg(n_max=10**9, seconds=4.0, rate=frac * capacity)
+        r["offered_req_per_s"] = round(frac * capacity, 1)
+        sweep.append(r)
+        log(f"serving sweep at {r['offered_req_per_s']}/s: p50 {r['p50_ms']} ms, p99 "
+            f"{r['p99_ms']} ms, {r['throughput_req_per_s']} req/s")
+    ok = [r for r in sweep if r["p99_ms"] <= P99_BOUND_MS
+          and r["throughput_req_per_s"] >= 0.97 * r["offered_req_per_s"]]
     st = srv.stats()
     srv.close()
-    lat = np.asarray(lat)
-    wall = (end_ns - base) * 1e-9
-    n_done = st["completed"] - stats0["completed"]
+    plain = bench_serve_plain(mix, ring)
     out = {
-        "workload": f"Poisson arrivals at {args.serve_rate:.0f}/s for {args.serve_seconds:.0f} s; 60% reuse "
-                    "one of 64 device-cached pairs, 40% build on arrival; per-axis vertices log-uniform "
-                    "in [33,1025]; 1-32 value arrays, FP32/FP64 50/50",
-        "requests": int(lat.size),
-        "p50_ms": round(float(np.percentile(lat, 50)), 3),
-        "p99_ms": round(float(np.percentile(lat, 99)), 3),
-        "mean_ms": round(float(lat.mean()), 3),
-        "throughput_req_per_s": round(n_done / wall, 1),
-        "offered_req_per_s": round(lat.size / args.serve_seconds, 1),
-        "builds": st["builds"] - stats0["builds"],
-        "cache_hits": st["cache_hits"] - stats0["cache_hits"],
-        "groups": st["groups"] - stats0["groups"],
+        "workload": "Poisson arrivals; 60% reuse one of 64 device-cached pairs, 40% build on "
+                    "arrival; per-axis vertices log-uniform in [33,1025]; 1-32 value arrays, "
+                    "FP32/FP64 50/50",
+        "offered": offered,
+        "capacity": cap,
+        "capacity_req_per_s": capacity,
+        "rate_sweep": sweep,
+        "p99_bound_ms": P99_BOUND_MS,
+        "max_rate_within_p99_bound": (max(r["offered_req_per_s"] for r in ok) if ok else None),
+        "plain": plain,
+        "served_vs_plain": round(capacity / plain["req_per_s"], 3),
         "max_group": st["max_group_seen"],
         "peak_admitted_GB": round(st["peak_admitted_bytes"] / 1e9, 3),
+        # the figures of the 2000/s run at the top level, as before
+        "requests": offered["requests"],
+        "p50_ms": offered["p50_ms"],
+        "p99_ms": offered["p99_ms"],
+        "throughput_req_per_s": offered["throughput_req_per_s"],
         "failed": st["failed"],
     }
-    log(f"serving: {out['requests']} requests, p50 {out['p50_ms']} ms, p99 {out['p99_ms']} ms, "
-        f"{out['throughput_req_per_s']} req/s, {out['groups']} groups, {out['builds']} builds")
+    log(f"serving: capacity {capacity} req/s, plain {plain['req_per_s']} req/s, "
+        f"x{out['served_vs_plain']}; max rate within p99 <= {P99_BOUND_MS} ms: "
+        f"{out['max_rate_within_p99_bound']}")
+    return out
+
+
+def bench_serve_plain(mix, ring, seconds: float = 6.0) -> dict:
+    """The serving mix one request per call on the plain C ABI: upload the
+    values, build the pair (fresh pairs; the reusable pairs' weights are
+    built once beforehand and kept), apply, download into pinned memory,
+    wait -- in sequence."""
+    import ctypes
+
+    from regridding_amd._conservative import build_2d
+    from regridding_amd._lib import DeviceArray, check, lib
+
+    L = lib()
+    kept = [build_2d(ci, co) for ci, co in mix.cached]
+    check(L.rg_device_sync())
+    n = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        reuse, j, x = mix.draw()
+        ci, co = mix.cached[j] if reuse else mix.fresh[j]
+        n_out = mix.cells(co)
+        if reuse:
+            h = kept[j]
+        else:
+            dxi, dyi = DeviceArray.from_host(ci[0]), DeviceArray.from_host(ci[1])
+            dxo, dyo = DeviceArray.from_host(co[0]), DeviceArray.from_host(co[1])
+            hp = ctypes.c_void_p()
+            check(L.rg_build_conservative_2d(dxi.ptr, dyi.ptr, ci[0].shape[0], ci[0].shape[1],
+                                             dxo.ptr, dyo.ptr, co[0].shape[0], co[0].shape[1],
+                                             None, None, ctypes.byref(hp), 0))
+        dx = DeviceArray.from_host(x)
+        dy = DeviceArray.empty((x.shape[0], n_out), x.dtype)
+        fn = L.rg_apply_f32 if x.dtype == np.float32 else L.rg_apply_f64
+        check(fn(h.ptr if reuse else hp, 0, dx.ptr, dy.ptr, x.shape[0], 0))
+        dy.to_host(out=ring[: dy.nbytes].view(x.dtype).reshape(x.shape[0], n_out))
+        if not reuse:
+            check(L.rg_weights_free(hp))
+        n += 1
+    check(L.rg_device_sync())
+    wall = time.perf_counter() - t0
+    for h in kept:
+        h.free()
+    out = {"requests": n, "seconds": round(wall, 2), "req_per_s": round(n / wall, 1)}
+    log(f"serving mix, plain one request per call: {out}")
     return out
 
 
@@ -689,46 +827,80 @@ def bench_inversion(args) -> dict:
     return out
 
 
-def cpu_baseline(ci, co, values, handle) -> dict:
-    """The C oracle (the reference's algorithm restated, same row-parallel
-    sweep, single-threaded coalesce, element-parallel apply) over the same
-    full pair on the host cores; its weights are also compared with the GPU's
-    bit for bit."""
+def cpu_cores() -> tuple[int, int]:
+    """(threads used, cores in the affinity mask).  The threads are every core
+    of the affinity mask, capped by the process's CPU share when the host
+    states one in OMP_NUM_THREADS (a shared GPU host gives each GPU a share
+    of its cores)."""
+    affinity = len(os.sched_getaffinity(0))
+    try:
+        share = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        share = 0
+    return (min(affinity, share) if share > 0 else affinity), affinity
+
+
+def cpu_baseline(ci, co, values, handle, y64, y32) -> dict:
+    """The C oracle (the reference's algorithm restated: row-parallel sweep,
+    single-threaded stable sort + reduceat, element-parallel apply) over the
+    same full pair on the host cores: one warm-up run, then the median of 3
+    (build and apply timed separately).  Its weights and FP64 outputs are
+    compared with the GPU's bit for bit, and the GPU's FP32 outputs with its
+    FP64 outputs.  The small 129x129 -> 101x101 pair: median of 5 on every
+    thread and on one."""
     from oracle import oracle
 
-    cores = min(16, len(os.sched_getaffinity(0)))
+    cores, affinity = cpu_cores()
     oracle.set_threads(cores)
     B = values.shape[0]
-    t0 = time.perf_counter()
-    flat = oracle.sweep_2d(ci, co)
-    t1 = time.perf_counter()
-    ii, io, w = oracle.coalesce(*flat)
-    t2 = time.perf_counter()
-    del flat
-    y = oracle.apply(ii, io, w, values.reshape(B, -1), handle.n_out)
-    t3 = time.perf_counter()
+    x = values.reshape(B, -1)
+
+    def run():
+        t0 = time.perf_counter()
+        flat = oracle.sweep_2d(ci, co)
+        t1 = time.perf_counter()
+        w = oracle.coalesce(*flat)
+        t2 = time.perf_counter()
+        del flat
+        y = oracle.apply(*w, x, handle.n_out)
+        t3 = time.perf_counter()
+        return (t1 - t0, t2 - t1, t3 - t2), w, y
+
+    _, (ii, io, w), y = run()  # warm-up; its results are the parity check
     gi, go, gw = handle.export()
     parity = bool(np.array_equal(gi, ii) and np.array_equal(go, io) and np.array_equal(gw, w))
-    log(f"cpu oracle ({cores} threads): sweep {t1 - t0:.2f} s, coalesce {t2 - t1:.2f} s, "
-        f"apply {t3 - t2:.2f} s; GPU weights bit-identical: {parity}")
-    del y
-    # the small CPU-runnable pair on one thread and on all: the scalar figure
+    del gi, go, gw
+    apply_parity = bool(np.array_equal(y64, y))
+    f32_err = f32_max_rel_err(y32, y)
+    del ii, io, w, y
+    runs = [run()[0] for _ in range(3)]
+    sweep, coal, app = (float(np.median([r[k] for r in runs])) for k in range(3))
+    total = float(np.median([sum(r) for r in runs]))
+    log(f"cpu oracle ({cores} threads): median of 3 after a warm-up: sweep {sweep:.2f} s, "
+        f"coalesce {coal:.2f} s, apply {app:.3f} s; GPU weights bit-identical: {parity}, "
+        f"FP64 apply bit-identical: {apply_parity}, FP32 max relative error {f32_err:.3g}")
+    # the small CPU-runnable pair on every thread and on one
     from regridding_amd import _util, workloads
 
     sgi, sgo, svals = workloads.plain_small()
     sci, sco, *_ = _util.normalize_coordinates(sgi, sgo, perturb=True)
+    sn_out = (sco[0].shape[0] - 1) * (sco[0].shape[1] - 1)
     small = {}
-    for nt in (1, cores):
+    for nt in (cores, 1):
         oracle.set_threads(nt)
-        best = None
-        for _ in range(3):
+        bt, at = [], []
+        for rep in range(6):  # the first is the warm-up
             u0 = time.perf_counter()
-            sii, sio, sw = oracle.coalesce(*oracle.sweep_2d(sci, sco))
-            oracle.apply(sii, sio, sw, svals.reshape(svals.shape[0], -1),
-                         (sco[0].shape[0] - 1) * (sco[0].shape[1] - 1))
-            dt = time.perf_counter() - u0
-            best = dt if best is None else min(best, dt)
-        small[f"seconds_{nt}_threads"] = round(best, 4)
+            sw = oracle.coalesce(*oracle.sweep_2d(sci, sco))
+            u1 = time.perf_counter()
+            oracle.apply(*sw, svals.reshape(svals.shape[0], -1), sn_out)
+            u2 = time.perf_counter()
+            if rep:
+                bt.append(u1 - u0)
+                at.append(u2 - u1)
+        small[f"{nt}_threads"] = {"build_s": round(float(np.median(bt)), 5),
+                                  "apply_s": round(float(np.median(at)), 5),
+                                  "pairs_per_s": round(1.0 / float(np.median(np.add(bt, at))), 2)}
     oracle.set_threads(cores)
     cpu_model = ""
     try:
@@ -741,18 +913,36 @@ def cpu_baseline(ci, co, values, handle) -> dict:
         pass
     log(f"cpu oracle, 129x129 -> 101x101 pair: {small}")
     return {
-        "value": round(1.0 / (t3 - t0), 6),
+        "value": round(1.0 / total, 6),
         "unit": "grid-pairs/s",
         "cores": cores,
+        "cores_in_affinity_mask": affinity,
         "kind": "port",
-        "sample": "the full pair: sweep (row-parallel), coalesce (1 thread), FP64 apply of all images "
-                  "(image-parallel), one run",
-        "seconds": {"sweep": round(t1 - t0, 3), "coalesce": round(t2 - t1, 3), "apply": round(t3 - t2, 3)},
+        "sample": "the full plain_large pair (2049x2049 -> 1537x1537, 16 FP64 images): sweep "
+                  "(row-parallel), coalesce (1 thread, as numpy), FP64 apply (image-parallel); "
+                  "median of 3 after a warm-up",
+        "seconds": {"build_sweep": round(sweep, 3), "build_coalesce": round(coal, 3),
+                    "build": round(sweep + coal, 3), "apply": round(app, 4),
+                    "total": round(total, 3)},
         "gpu_weights_bit_identical": parity,
+        "gpu_apply_f64_bit_identical": apply_parity,
+        "gpu_apply_f32_max_rel_err": f32_err,
         "cpu_model": cpu_model,
-        "small_pair": dict(small, sample="129x129 -> 101x101 pair, 4 images: sweep + coalesce + "
-                                         "apply, best of 3"),
+        "small_pair": dict(small, sample="129x129 -> 101x101 pair, 4 FP64 images: build (sweep + "
+                                         "coalesce) and apply, median of 5 after a warm-up"),
     }
+
+
+def f32_max_rel_err(y32, y64) -> float:
+    """max |y32 - y64| / |y64| over outputs with y64 != 0 (inf when an output
+    that is exactly zero in FP64 is not zero in FP32)."""
+    y32 = np.asarray(y32, dtype=np.float64)
+    nz = y64 != 0
+    if y32[~nz].any():
+        return float("inf")
+    if not nz.any():
+        return 0.0
+    return float(np.max(np.abs(y32[nz] - y64[nz]) / np.abs(y64[nz])))
 
 
 if __name__ == "__main__":
