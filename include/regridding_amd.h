@@ -249,6 +249,19 @@ int rg_find_cells_2d(const double* x, const double* y, int64_t nx, int64_t ny,
                      const double* px, const double* py, int64_t n,
                      int64_t* cell_ij, void* stream);
 
+/* No counterpart in the reference: rg_find_cells_2d for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and results, bit for
+ * bit; it calls no allocator.  Its 8 bytes of scratch per point live in a
+ * fixed device array of RG_FIND_CELLS_NOALLOC_MAX points per GPU (16 MiB,
+ * held until the process ends); larger n fails with RG_ERR_ARGUMENT (search
+ * in slices).  Every call on one GPU shares that array: calls that overlap
+ * on one GPU race, so issue them on one stream. */
+#define RG_FIND_CELLS_NOALLOC_MAX (1 << 21)
+int rg_find_cells_2d_noalloc(const double* x, const double* y, int64_t nx,
+                             int64_t ny, const double* px, const double* py,
+                             int64_t n, int64_t* cell_ij, void* stream);
+
 #define RG_FIND_BRUTE 0
 #define RG_FIND_SEARCHSORTED 1
 

@@ -207,6 +207,7 @@ SIGNATURES = {
     "rg_apply_f64": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _i64, _vp]),
     "rg_apply_f32": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _i64, _vp]),
     "rg_find_cells_2d": (ctypes.c_int, [_vp, _vp, _i64, _i64, _vp, _vp, _i64, _vp, _vp]),
+    "rg_find_cells_2d_noalloc": (ctypes.c_int, [_vp, _vp, _i64, _i64, _vp, _vp, _i64, _vp, _vp]),
     "rg_find_indices_1d": (ctypes.c_int, [_vp, _i64, _vp, _i64, _i64, ctypes.c_int, _i64, _vp, _vp]),
     "rg_inject_failure": (ctypes.c_int, [ctypes.c_char_p]),
 }

@@ -83,6 +83,10 @@ struct DevBuf {
 // Pinned host scratch for small synchronous device->host reads.
 int read_scalars(const void* dev, void* host, size_t bytes, hipStream_t s);
 
+// Device address, on the current device, of the fixed scratch array of
+// rg_find_cells_2d_noalloc (looked up once per device and kept).
+int find_cells_scratch(int64_t** out);
+
 // ---- scans ---------------------------------------------------------------
 
 // out[0..n] = exclusive prefix sum of in[0..n), out[n] = total.  `in` and

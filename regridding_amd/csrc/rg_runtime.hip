@@ -405,7 +405,9 @@ int rg_init(int device)
                                          std::to_string(n) + " devices)");
     RG_HIP(hipSetDevice(device));
     RG_HIP(hipFree(nullptr));  // create the context now
-    return RG_OK;
+    // device address of the no-allocation cell search's fixed scratch
+    int64_t* scratch = nullptr;
+    return find_cells_scratch(&scratch);
 }
 
 int rg_device_count(int* count)

@@ -5,6 +5,9 @@
 // the cell-area entry point.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <string>
+
 #include "rg_internal.hpp"
 #include "rg_wave.hpp"
 
@@ -101,6 +104,74 @@ __global__ void k_find_sorted_1d(const double* __restrict__ xin, int64_t m,
     res[t] = r;
 }
 
+namespace {
+
+int check_find_cells(const double* x, const double* y, int64_t nx, int64_t ny,
+                     const double* px, const double* py, int64_t* cell_ij)
+{
+    if (!x || !y || !px || !py || !cell_ij)
+        return fail(RG_ERR_ARGUMENT, "NULL argument");
+    if (nx < 2 || ny < 2)
+        return fail(RG_ERR_ARGUMENT, "grid needs at least 2x2 vertices");
+    return RG_OK;
+}
+
+// The two kernels of the cell search; `cell` holds one int64 per point
+// between them (every entry written by the first before the second reads it).
+int launch_find_cells(const double* x, const double* y, int64_t nx, int64_t ny,
+                      const double* px, const double* py, int64_t n,
+                      int64_t* cell, int64_t* cell_ij, hipStream_t s)
+{
+    k_cells_secant<<<grid_for(n, 128), 128, 0, s>>>(x, y, nx, ny, px, py, n, cell);
+    k_cells_finish<<<grid_for(n * 64, 256), 256, 0, s>>>(x, y, nx, ny, px, py, n,
+                                                         cell, cell_ij);
+    RG_HIP(hipGetLastError());
+    return RG_OK;
+}
+
+}  // namespace
+
+// ---- the cell search without allocation -------------------------------------
+//
+// rg_find_cells_2d_noalloc keeps its per-point scratch in this fixed array
+// (one copy per GPU, RG_FIND_CELLS_NOALLOC_MAX * 8 bytes = 16 MiB, held
+// until the process ends).  Host code never passes the symbol itself to a
+// kernel or copy: on the host it names a host-side stand-in, not device
+// memory.  Its device address is looked up once per device, with that device
+// current, and kept; calls on one GPU share the array, so calls that
+// overlap on one GPU race (issue them on one stream).  Nothing in it needs
+// clearing: the secant kernel writes every entry the finishing kernel reads.
+static __device__ __align__(256) int64_t scratch[RG_FIND_CELLS_NOALLOC_MAX];
+
+namespace {
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+int64_t* g_scratch_addr[SCRATCH_MAX_DEVICES] = {};
+
+}  // namespace
+
+int find_cells_scratch(int64_t** out)
+{
+    *out = nullptr;
+    int dev = 0;
+    RG_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES)
+        return fail(RG_ERR_ARGUMENT, "device " + std::to_string(dev) +
+                                         " beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(RG_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                        hipGetErrorString(e));
+        g_scratch_addr[dev] = (int64_t*)p;
+    }
+    *out = g_scratch_addr[dev];
+    return RG_OK;
+}
+
 }  // namespace rg
 
 using namespace rg;
@@ -120,19 +191,29 @@ int rg_find_cells_2d(const double* x, const double* y, int64_t nx, int64_t ny,
                      int64_t* cell_ij, void* stream)
 {
     if (n == 0) return RG_OK;
-    if (!x || !y || !px || !py || !cell_ij)
-        return fail(RG_ERR_ARGUMENT, "NULL argument");
-    if (nx < 2 || ny < 2)
-        return fail(RG_ERR_ARGUMENT, "grid needs at least 2x2 vertices");
+    RG_TRY(check_find_cells(x, y, nx, ny, px, py, cell_ij));
     hipStream_t s = (hipStream_t)stream;
     DevBuf<int64_t> cell;
     RG_TRY(cell.alloc(n, s));
-    k_cells_secant<<<grid_for(n, 128), 128, 0, s>>>(x, y, nx, ny, px, py, n,
-                                                    cell.p);
-    k_cells_finish<<<grid_for(n * 64, 256), 256, 0, s>>>(x, y, nx, ny, px, py,
-                                                         n, cell.p, cell_ij);
-    RG_HIP(hipGetLastError());
-    return RG_OK;
+    return launch_find_cells(x, y, nx, ny, px, py, n, cell.p, cell_ij, s);
+}
+
+int rg_find_cells_2d_noalloc(const double* x, const double* y, int64_t nx,
+                             int64_t ny, const double* px, const double* py,
+                             int64_t n, int64_t* cell_ij, void* stream)
+{
+    if (n == 0) return RG_OK;
+    RG_TRY(check_find_cells(x, y, nx, ny, px, py, cell_ij));
+    if (n < 0 || n > RG_FIND_CELLS_NOALLOC_MAX)
+        return fail(RG_ERR_ARGUMENT,
+                    "rg_find_cells_2d_noalloc: " + std::to_string(n) +
+                        " points exceed its fixed scratch of " +
+                        std::to_string((long long)RG_FIND_CELLS_NOALLOC_MAX) +
+                        " points per call; search them in slices");
+    int64_t* cell = nullptr;
+    RG_TRY(find_cells_scratch(&cell));
+    return launch_find_cells(x, y, nx, ny, px, py, n, cell, cell_ij,
+                             (hipStream_t)stream);
 }
 
 int rg_find_indices_1d(const double* x_input, int64_t m,
