@@ -1134,17 +1134,26 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
     if (lane == 0) group_mode[grp] = packed ? 0 : 1;
 }
 
-// Groups beyond the LDS share, one wavefront each (grid-stride over the
-// list).  The cells' counts give their cell-major offsets in `red` (the
-// group is marked unpacked); the cells are then taken in runs of
-// consecutive cells that fit the LDS share.  For each run the wave reads
-// the group's cell bytes and places the records of the run's cells in LDS,
-// cell-major; every record finds its place in its cell's key order by
-// counting the smaller keys of its cell (the keys are unique: they hold the
-// emission ordinal), and each lane reduces one cell at its own offset.  A
-// single cell beyond the share is copied cell-major into `red` and left to
-// k_bucket_long.
+// Groups beyond the LDS share, one wavefront each.  The waves are resident
+// and pull the next group of the list from a counter (a work queue: the
+// heavy groups differ in size by orders of magnitude, so a fixed
+// assignment leaves waves idle behind the one that drew the largest).  The
+// cells' counts give their cell-major offsets in `red` (the group is marked
+// unpacked); the cells are then taken in runs of consecutive cells that fit
+// the LDS share.  For each run the wave places the records of the run's
+// cells in LDS, cell-major; every record finds its place in its cell's key
+// order by counting the smaller keys of its cell (the keys are unique: they
+// hold the emission ordinal), and each lane reduces one cell at its own
+// offset.  A single cell beyond the share is copied cell-major into `red`
+// and left to k_bucket_long.
+//
+// Each run lists its records' sources in LDS first (the group's cell bytes
+// read HEAVY_UNROLL at a time per lane), then loads the records with
+// HEAVY_UNROLL loads per lane in flight, each record written over its own
+// source slot: two short dependent chains per run instead of one per
+// record.
 constexpr int HEAVY_CAP = RED_CAP;  // LDS entries per wavefront
+constexpr int HEAVY_UNROLL = 2;  // more raises the kernel past 256 registers
 
 __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_heavy(
     const int64_t* __restrict__ bucket_off, int64_t n_groups,
@@ -1154,8 +1163,9 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_heavy(
     const double* __restrict__ wi, int32_t* __restrict__ nnz_row,
     int32_t* __restrict__ group_mode, int32_t* __restrict__ long_list,
     int32_t* __restrict__ n_long, const int32_t* __restrict__ heavy_list,
-    const int32_t* __restrict__ n_heavy, const PairCells* __restrict__ pcs,
-    const int32_t* __restrict__ pair_of_group, int32_t* __restrict__ hist_out_all)
+    const int32_t* __restrict__ n_heavy, int32_t* __restrict__ heavy_next,
+    const PairCells* __restrict__ pcs, const int32_t* __restrict__ pair_of_group,
+    int32_t* __restrict__ hist_out_all)
 {
     __shared__ uint64_t sk[RED_WAVES][HEAVY_CAP];
     __shared__ double sv[RED_WAVES][HEAVY_CAP];
@@ -1165,11 +1175,15 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_heavy(
     int32_t* C = scnt[wid];
     uint64_t* K = sk[wid];
     double* V = sv[wid];
+    int64_t* SRC = reinterpret_cast<int64_t*>(sv[wid]);  // a run's record sources
     uint8_t* SC = sc[wid];
     constexpr int U = HEAVY_CAP / 64;
     const int32_t nh = *n_heavy;
-    for (int64_t h = (int64_t)blockIdx.x * RED_WAVES + wid; h < nh;
-         h += (int64_t)gridDim.x * RED_WAVES) {  // uniform per wavefront
+    for (;;) {
+        int32_t h = 0;
+        if (lane == 0) h = atomicAdd(heavy_next, 1);
+        h = __shfl(h, 0, 64);
+        if (h >= nh) break;  // uniform per wavefront
         const int64_t grp = heavy_list[h];
         const int64_t i = grp * 64 + lane;
         const int64_t base = group_off[grp];
@@ -1179,14 +1193,14 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_heavy(
             hist_out_all + pcs[__builtin_amdgcn_readfirstlane(pair_of_group[grp])].out_gbase;
         C[lane] = 0;
         wave_sync_lds();
-        for (int64_t q0 = lane; q0 < total; q0 += 256) {
-            int32_t c4[4];
+        for (int64_t q0 = lane; q0 < total; q0 += 64 * HEAVY_UNROLL) {
+            int32_t cb[HEAVY_UNROLL];
 #pragma unroll
-            for (int u = 0; u < 4; u++)
-                c4[u] = q0 + 64 * u < total ? rcell[G.src(q0 + 64 * u)] : -1;
+            for (int u = 0; u < HEAVY_UNROLL; u++)
+                cb[u] = q0 + 64 * u < total ? rcell[G.src(q0 + 64 * u)] : -1;
 #pragma unroll
-            for (int u = 0; u < 4; u++)
-                if (c4[u] >= 0) atomicAdd(&C[c4[u]], 1);
+            for (int u = 0; u < HEAVY_UNROLL; u++)
+                if (cb[u] >= 0) atomicAdd(&C[cb[u]], 1);
         }
         wave_sync_lds();
         const int32_t n = C[lane];
@@ -1217,26 +1231,36 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_heavy(
             const int32_t cnt = __shfl(s + n, d - 1, 64) - sb;
             C[lane] = s - sb;  // placement cursors of the run's cells
             wave_sync_lds();
-            // the run's records, cell-major in LDS
-            for (int64_t q0 = 0; q0 < total; q0 += 128) {  // uniform trip count
-                const int64_t qa = q0 + lane, qb = qa + 64;
-                const int64_t sa = qa < total ? G.src(qa) : -1;
-                const int64_t sb2 = qb < total ? G.src(qb) : -1;
-                const int32_t ca = sa >= 0 ? rcell[sa] : -1;
-                const int32_t cb = sb2 >= 0 ? rcell[sb2] : -1;
-                if (ca >= c && ca < d) {
-                    const KV r = rec[sa];
-                    const int32_t p = atomicAdd(&C[ca], 1);
-                    K[p] = r.key;
-                    V[p] = r.val;
-                    SC[p] = (uint8_t)ca;
+            // the run's records, cell-major in LDS: their sources first ...
+            for (int64_t q0 = lane; q0 < total; q0 += 64 * HEAVY_UNROLL) {
+                int32_t cb[HEAVY_UNROLL];
+#pragma unroll
+                for (int u = 0; u < HEAVY_UNROLL; u++)
+                    cb[u] = q0 + 64 * u < total ? rcell[G.src(q0 + 64 * u)] : -1;
+#pragma unroll
+                for (int u = 0; u < HEAVY_UNROLL; u++)
+                    if (cb[u] >= c && cb[u] < d) {
+                        const int32_t p = atomicAdd(&C[cb[u]], 1);
+                        SRC[p] = G.src(q0 + 64 * u);
+                        SC[p] = (uint8_t)cb[u];
+                    }
+            }
+            wave_sync_lds();
+            // ... then the records, each written over its own source slot
+            for (int32_t e0 = lane; e0 < cnt; e0 += 64 * HEAVY_UNROLL) {
+                KV r[HEAVY_UNROLL];
+#pragma unroll
+                for (int u = 0; u < HEAVY_UNROLL; u++) {
+                    const int32_t e = e0 + 64 * u;
+                    if (e < cnt) r[u] = rec[SRC[e]];
                 }
-                if (cb >= c && cb < d) {
-                    const KV r = rec[sb2];
-                    const int32_t p = atomicAdd(&C[cb], 1);
-                    K[p] = r.key;
-                    V[p] = r.val;
-                    SC[p] = (uint8_t)cb;
+#pragma unroll
+                for (int u = 0; u < HEAVY_UNROLL; u++) {
+                    const int32_t e = e0 + 64 * u;
+                    if (e < cnt) {
+                        K[e] = r[u].key;
+                        V[e] = r[u].val;
+                    }
                 }
             }
             wave_sync_lds();
@@ -2035,7 +2059,8 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
     RG_TRY(hist.alloc(4 * n_groups, s));
     // anomalies, fixups, failed/overflow, long runs, restaged chunks, heavy
     // groups; words 6-7: the triples beyond the staging capacity (64-bit);
-    // word 8: capacity bits of a capacity-mode build
+    // word 8: capacity bits of a capacity-mode build; word 9: the heavy
+    // groups' work-queue cursor
     RG_TRY(counters.alloc(12, s));
     RG_TRY(fill_zero(hist.p, 4 * n_groups, s));
     RG_TRY(fill_zero(counters.p, 12, s));
@@ -2175,10 +2200,11 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
                                               counters.p + 3, heavy_list.p, counters.p + 5,
                                               d_pcs, pair_of_ing.p, hist_out.p, T,
                                               counters.p + 8);
+    // (resident waves: four blocks of two waves per CU fit the LDS)
     k_reduce_heavy<<<1024, RED_WAVES * 64, 0, s>>>(
         bucket_off.p, n_groups, group_off.p, rec.p, rcell.p, red.p, cell_off.p, vol.p,
         wi_all.p, nnz_row.p, group_mode.p, long_list.p, counters.p + 3, heavy_list.p,
-        counters.p + 5, d_pcs, pair_of_ing.p, hist_out.p);
+        counters.p + 5, counters.p + 9, d_pcs, pair_of_ing.p, hist_out.p);
     k_bucket_long<<<256, 64, 0, s>>>(cell_off.p, red.p, vol.p, wi_all.p, long_list.p,
                                    counters.p + 3, nnz_row.p, d_pcs, pair_of_ing.p,
                                    hist_out.p);
