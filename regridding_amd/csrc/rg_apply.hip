@@ -94,7 +94,13 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_apply(
 // row load per box row, and gathers from LDS.
 
 constexpr int LDS_BYTES = 64 * 1024;  // staged patch per block
-constexpr int STAGE_UNROLL = 8;
+// Values each lane loads before its LDS stores in a staging step (measured:
+// more per lane gains nothing for either precision, fewer loses for FP64).
+template <class T>
+constexpr int stage_unroll()
+{
+    return 8;
+}
 constexpr int64_t PLAN_MIN_ROWS = 65536;
 
 // a / b from a float reciprocal and one correction step (the float quotient
@@ -204,16 +210,17 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_apply_planned(
     const bool staged = cells > 0 && cells * nd * (int64_t)sizeof(T) <= lds_bytes;
     if (staged) {
         // The box of nd images, flattened (image, row, column) over the
-        // block's lanes; each lane issues STAGE_UNROLL loads before its LDS
+        // block's lanes; each lane issues stage_unroll<T>() loads before its LDS
         // stores, so the staging costs about one memory latency instead of
         // one per box row.
         const int32_t ncell = (int32_t)cells, total = ncell * (int32_t)nd;
         const float inv_cell = 1.0f / (float)ncell, inv_cols = 1.0f / (float)cols;
         const T* xb = x + d0 * ncols + (int64_t)i0 * src_gy + j0;
-        for (int32_t e0 = threadIdx.x; e0 < total; e0 += APPLY_BLOCK * STAGE_UNROLL) {
-            T v[STAGE_UNROLL];
+        constexpr int SU = stage_unroll<T>();
+        for (int32_t e0 = threadIdx.x; e0 < total; e0 += APPLY_BLOCK * SU) {
+            T v[SU];
 #pragma unroll
-            for (int u = 0; u < STAGE_UNROLL; u++) {
+            for (int u = 0; u < SU; u++) {
                 const int32_t e = e0 + u * APPLY_BLOCK;
                 if (e < total) {
                     const int32_t m = fast_div(e, ncell, inv_cell);
@@ -224,7 +231,7 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_apply_planned(
                 }
             }
 #pragma unroll
-            for (int u = 0; u < STAGE_UNROLL; u++) {
+            for (int u = 0; u < SU; u++) {
                 const int32_t e = e0 + u * APPLY_BLOCK;
                 if (e < total) patch[e] = v[u];
             }
@@ -334,10 +341,11 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_apply_rounds(
         if (fits) {
             const int32_t total = ncell * (int32_t)nd;
             const T* xb = x + d0 * ncols + (int64_t)i0 * src_gy + j0;
-            for (int32_t e0 = threadIdx.x; e0 < total; e0 += APPLY_BLOCK * STAGE_UNROLL) {
-                T v[STAGE_UNROLL];
+            constexpr int SU = stage_unroll<T>();
+            for (int32_t e0 = threadIdx.x; e0 < total; e0 += APPLY_BLOCK * SU) {
+                T v[SU];
 #pragma unroll
-                for (int u = 0; u < STAGE_UNROLL; u++) {
+                for (int u = 0; u < SU; u++) {
                     const int32_t ee = e0 + u * APPLY_BLOCK;
                     if (ee < total) {
                         const int32_t m = fast_div(ee, ncell, inv_cell);
@@ -348,7 +356,7 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_apply_rounds(
                     }
                 }
 #pragma unroll
-                for (int u = 0; u < STAGE_UNROLL; u++) {
+                for (int u = 0; u < SU; u++) {
                     const int32_t ee = e0 + u * APPLY_BLOCK;
                     if (ee < total) patch[ee] = v[u];
                 }
@@ -606,6 +614,27 @@ int ensure_f32(rg_weights* W, hipStream_t s)
     return RG_OK;
 }
 
+std::mutex g_use_mu;
+
+}  // namespace
+
+int note_use(rg_weights* W, hipStream_t s)
+{
+    if (s == W->home) return RG_OK;
+    std::lock_guard<std::mutex> lk(g_use_mu);
+    hipEvent_t ev = nullptr;
+    for (auto& u : W->uses)
+        if (u.first == s) ev = u.second;
+    if (!ev) {
+        RG_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        W->uses.emplace_back(s, ev);
+    }
+    RG_HIP(hipEventRecord(ev, s));
+    return RG_OK;
+}
+
+namespace {
+
 int check_apply(rg_weights* W, const void* vin, void* vout, int64_t batch)
 {
     if (!W) return fail(RG_ERR_ARGUMENT, "weights handle is NULL");
@@ -696,6 +725,7 @@ int launch_jobs_common(const rg_apply_job* jobs, int64_t njobs, bool f32,
         k_apply_jobs<double><<<(unsigned)nblocks, APPLY_BLOCK, 0, s>>>(
             d_jobs, d_prefix, (int)table.size(), nullptr);
     RG_HIP(hipGetLastError());
+    for (int64_t k = 0; k < njobs; k++) RG_TRY(note_use(jobs[k].weights, s));
     return RG_OK;  // dt returns to the stream's pool after the kernel
 }
 
@@ -726,13 +756,15 @@ int rg_apply_f64(rg_weights_t w, int transpose, const double* values_input,
     RG_TRY(check_apply(w, values_input, values_output, batch));
     hipStream_t s = (hipStream_t)stream;
     if (transpose)
-        return launch_apply<double>(w, 1, w->rowptr_in, w->col_in, w->val_in,
+        RG_TRY(launch_apply<double>(w, 1, w->rowptr_in, w->col_in, w->val_in,
                                     values_input, values_output, w->n_in,
                                     w->n_out, w->shape_in, w->shape_out[1],
-                                    batch, s);
-    return launch_apply<double>(w, 0, w->rowptr_out, w->col_out, w->val_out,
-                                values_input, values_output, w->n_out, w->n_in,
-                                w->shape_out, w->shape_in[1], batch, s);
+                                    batch, s));
+    else
+        RG_TRY(launch_apply<double>(w, 0, w->rowptr_out, w->col_out, w->val_out,
+                                    values_input, values_output, w->n_out, w->n_in,
+                                    w->shape_out, w->shape_in[1], batch, s));
+    return note_use(w, s);
 }
 
 int rg_apply_f32(rg_weights_t w, int transpose, const float* values_input,
@@ -742,13 +774,15 @@ int rg_apply_f32(rg_weights_t w, int transpose, const float* values_input,
     hipStream_t s = (hipStream_t)stream;
     RG_TRY(ensure_f32(w, s));
     if (transpose)
-        return launch_apply<float>(w, 1, w->rowptr_in, w->col_in, w->val_in_f32,
+        RG_TRY(launch_apply<float>(w, 1, w->rowptr_in, w->col_in, w->val_in_f32,
                                    values_input, values_output, w->n_in,
                                    w->n_out, w->shape_in, w->shape_out[1],
-                                   batch, s);
-    return launch_apply<float>(w, 0, w->rowptr_out, w->col_out, w->val_out_f32,
-                               values_input, values_output, w->n_out, w->n_in,
-                               w->shape_out, w->shape_in[1], batch, s);
+                                   batch, s));
+    else
+        RG_TRY(launch_apply<float>(w, 0, w->rowptr_out, w->col_out, w->val_out_f32,
+                                   values_input, values_output, w->n_out, w->n_in,
+                                   w->shape_out, w->shape_in[1], batch, s));
+    return note_use(w, s);
 }
 
 int rg_apply_batched_f64(const rg_apply_job* jobs, int64_t njobs, void* stream)

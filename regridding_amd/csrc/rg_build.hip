@@ -2322,6 +2322,7 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
         const int64_t eb = hbases[2 * k];
         rg_weights* W = new rg_weights();
         W->device = device;
+        W->home = s;
         W->n_in = P.n_in;
         W->n_out = P.n_out;
         W->nnz = hbases[2 * k + 2] - eb;
@@ -2444,6 +2445,7 @@ int graph_weights_copy(GraphImpl* G, int64_t k, rg_weights** out, hipStream_t s)
     RG_HIP(hipStreamSynchronize(s));
     rg_weights* W = new rg_weights();
     W->device = G->device;
+    W->home = s;
     W->n_in = P.n_in;
     W->n_out = P.n_out;
     W->nnz = nnz;
@@ -2540,6 +2542,12 @@ namespace rg {
 void weights_release(rg_weights* W)
 {
     if (!W) return;
+    // applies still queued on streams other than the handle's own read its
+    // arrays: the memory returns to its own stream's pool only after them
+    for (auto& u : W->uses) {
+        (void)hipEventSynchronize(u.second);
+        (void)hipEventDestroy(u.second);
+    }
     if (!W->arena) {
         dev_free(W->rowptr_in);
         dev_free(W->col_in);
@@ -2704,6 +2712,7 @@ int import_triples(const int64_t* ii, const int64_t* io, const double* w,
     if (nnz > 0 && (!ii || !io || !w))
         return fail(RG_ERR_ARGUMENT, "weight array is NULL");
     rg_weights* W = new rg_weights();
+    W->home = s;
     W->n_in = n_in;
     W->n_out = n_out;
     W->nnz = nnz;

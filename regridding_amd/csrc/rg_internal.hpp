@@ -180,4 +180,16 @@ struct rg_weights {
     rg_build_stats stats = {};
     // set when the six CSR arrays are slices of a batched build's buffers
     std::shared_ptr<rg_arena> arena;
+    // The stream the handle was made on, and every other stream that has
+    // applied it with an event recorded after its latest apply there: the
+    // device memory goes back to the pool of the stream it was allocated
+    // on, so release first waits for applies still queued elsewhere.
+    hipStream_t home = nullptr;
+    std::vector<std::pair<hipStream_t, hipEvent_t>> uses;
 };
+
+namespace rg {
+// Notes an apply of W on stream s (after its launch): records s's event
+// unless s is the handle's own stream.
+int note_use(rg_weights* W, hipStream_t s);
+}  // namespace rg

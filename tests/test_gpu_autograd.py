@@ -140,3 +140,23 @@ def test_inversion_reduces_misfit_and_error():
     err0 = float(torch.linalg.norm(x0 - truth))
     err = float(torch.linalg.norm(x - truth))
     assert err < 0.5 * err0
+
+
+def test_free_waits_for_applies_on_another_stream():
+    """Weights applied on a side stream and freed at once: the free returns
+    only after the side stream's applies, whose results are intact (the CSR
+    memory goes back to the pool of the stream it was built on)."""
+    ci, co = _pair(513, 385)
+    h = build_2d(ci, co)
+    x = torch.tensor(np.random.default_rng(8).poisson(30, (16, h.n_in)).astype(np.float64),
+                     device="cuda")
+    expect = regrid_apply(h, x)
+    torch.cuda.synchronize()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        ys = [regrid_apply(h, x) for _ in range(40)]
+    h.free()  # no synchronisation in between
+    assert side.query(), "the free returned while applies on the side stream were queued"
+    for y in ys:
+        assert torch.equal(y, expect)
