@@ -524,7 +524,8 @@ def _drive(srv, mix, ring, n_max, seconds, rate=None, window=None):
     times of `rate` per second (open loop), or keeping `window` requests in
     flight (closed loop).  Results land in slices of the pinned byte `ring`,
     released in submission order.  Returns (latencies in ms, completions,
-    elapsed seconds from the first arrival to the last completion)."""
+    elapsed seconds from the first arrival to the last completion, bytes
+    moved host -> device and device -> host)."""
     import threading
     from collections import deque
 
@@ -587,6 +588,7 @@ def _drive(srv, mix, ring, n_max, seconds, rate=None, window=None):
     base = clock_ns() + 50_000_000
     t_arr = 0.0
     n = 0
+    moved = {"h2d": 0, "d2h": 0}  # bytes each request moves over PCIe
     try:
         while n < n_max:
             if rate is not None:
@@ -621,6 +623,9 @@ def _drive(srv, mix, ring, n_max, seconds, rate=None, window=None):
                                 result=res, arrival_ns=at)
             else:
                 tk = srv.submit(x, ci, co, key=0, result=res, arrival_ns=at)
+                moved["h2d"] += 16 * (ci[0].size + co[0].size)  # the grids
+            moved["h2d"] += x.nbytes
+            moved["d2h"] += nb
             n += 1
             with lock:
                 check_failed()
@@ -632,7 +637,7 @@ def _drive(srv, mix, ring, n_max, seconds, rate=None, window=None):
             lock.notify_all()
         th.join()
     check_failed()
-    return np.asarray(lat), n, (state["last_ns"] - base) * 1e-9
+    return np.asarray(lat), n, (state["last_ns"] - base) * 1e-9, moved
 
 
 def _lat_stats(lat):
@@ -641,17 +646,20 @@ def _lat_stats(lat):
             "mean_ms": round(float(lat.mean()), 3)}
 
 
+SERVE_SWEEP_RATES = (1000, 1500, 2000, 2500, 3000, 3500)
+
+
 def bench_serve(args) -> dict:
     """The serving loop (continuous batching) under the serving workload:
 
     * `offered`: Poisson arrivals at --serve-rate (2000/s) for --serve-seconds
       (30 s): p50/p99 latency (scheduled arrival -> result in host memory) and
       throughput;
-    * `capacity`: a closed loop keeping 256 requests in flight for 8 s: the
-      loop's maximum sustained request rate;
-    * `rate_sweep`: Poisson arrivals at fractions of that capacity, 4 s each:
-      the highest offered rate whose p99 stays within P99_BOUND_MS and whose
-      throughput keeps up with the arrivals;
+    * `rate_sweep`: Poisson arrivals at rising rates (SERVE_SWEEP_RATES), 4 s
+      each: the highest offered rate whose p99 stays within P99_BOUND_MS and
+      whose throughput keeps up with the arrivals;
+    * `capacity`: then a closed loop keeping 256 requests in flight for 8 s:
+      the loop's maximum sustained request rate;
     * `plain`: the same request mix served one request per call (build a
       fresh pair with rg_build_conservative_2d, apply with rg_apply_*,
       synchronous copies; the 64 reusable pairs' weights kept): the
@@ -676,32 +684,45 @@ def bench_serve(args) -> dict:
 
     def leg(**kw):
         s0 = srv.stats()
-        lat, n, wall = _drive(srv, mix, ring, **kw)
+        lat, n, wall, moved = _drive(srv, mix, ring, **kw)
         s1 = srv.stats()
         d = {k: s1[k] - s0[k] for k in ("completed", "failed", "builds", "cache_hits", "groups")}
         out = {"requests": int(n), **_lat_stats(lat),
                "throughput_req_per_s": round(d["completed"] / wall, 1),
                "builds": d["builds"], "cache_hits": d["cache_hits"], "groups": d["groups"],
-               "failed": d["failed"]}
+               "failed": d["failed"],
+               "h2d_GBps": round(moved["h2d"] / wall / 1e9, 2),
+               "d2h_GBps": round(moved["d2h"] / wall / 1e9, 2)}
         return out
+
+    import ctypes
+
+    from regridding_amd._lib import lib
+
+    def free_gb():
+        fr, tot = ctypes.c_size_t(), ctypes.c_size_t()
+        lib().rg_mem_info(ctypes.byref(fr), ctypes.byref(tot))
+        return round(fr.value / 2**30, 1)
 
     offered = leg(n_max=10**9, seconds=args.serve_seconds, rate=args.serve_rate)
     offered["offered_req_per_s"] = round(offered["requests"] / args.serve_seconds, 1)
-    log(f"serving at {args.serve_rate:.0f}/s: {offered}")
+    log(f"serving at {args.serve_rate:.0f}/s: {offered}; free device memory {free_gb()} GiB")
+    # rising offered rates (open loop), then the closed loop that saturates
+    sweep = []
+    for rate in SERVE_SWEEP_RATES:
+        r = leg(n_max=10**9, seconds=4.0, rate=rate)
+        r["offered_req_per_s"] = float(rate)
+        sweep.append(r)
+        log(f"serving sweep at {rate}/s: p50 {r['p50_ms']} ms, p99 {r['p99_ms']} ms, "
+            f"{r['throughput_req_per_s']} req/s; free device memory {free_gb()} GiB")
     cap = leg(n_max=10**9, seconds=8.0, window=256)
     capacity = cap["throughput_req_per_s"]
-    log(f"serving capacity (256 in flight): {cap}")
-    sweep = []
-    for frac in (0.5, 0.7, 0.85, 1.0):
-        r = leg(n_max=10**9, seconds=4.0, rate=frac * capacity)
-        r["offered_req_per_s"] = round(frac * capacity, 1)
-        sweep.append(r)
-        log(f"serving sweep at {r['offered_req_per_s']}/s: p50 {r['p50_ms']} ms, p99 "
-            f"{r['p99_ms']} ms, {r['throughput_req_per_s']} req/s")
+    log(f"serving capacity (256 in flight): {cap}; free device memory {free_gb()} GiB")
     ok = [r for r in sweep if r["p99_ms"] <= P99_BOUND_MS
           and r["throughput_req_per_s"] >= 0.97 * r["offered_req_per_s"]]
     st = srv.stats()
     srv.close()
+    pcie = pcie_bandwidth(ring)
     plain = bench_serve_plain(mix, ring)
     out = {
         "workload": "Poisson arrivals; 60% reuse one of 64 device-cached pairs, 40% build on "
@@ -715,6 +736,10 @@ def bench_serve(args) -> dict:
         "max_rate_within_p99_bound": (max(r["offered_req_per_s"] for r in ok) if ok else None),
         "plain": plain,
         "served_vs_plain": round(capacity / plain["req_per_s"], 3),
+        # the loop moves every request's values in and results out over
+        # PCIe: its roofline is the copy engines' bandwidth
+        "pcie": pcie,
+        "capacity_pcie_frac": round((cap["h2d_GBps"] + cap["d2h_GBps"]) / pcie["both_GBps"], 3),
         "max_group": st["max_group_seen"],
         "peak_admitted_GB": round(st["peak_admitted_bytes"] / 1e9, 3),
         # the figures of the 2000/s run at the top level, as before
@@ -727,6 +752,51 @@ def bench_serve(args) -> dict:
     log(f"serving: capacity {capacity} req/s, plain {plain['req_per_s']} req/s, "
         f"x{out['served_vs_plain']}; max rate within p99 <= {P99_BOUND_MS} ms: "
         f"{out['max_rate_within_p99_bound']}")
+    return out
+
+
+def pcie_bandwidth(ring, gib: int = 1, reps: int = 4) -> dict:
+    """Host <-> device copy bandwidth between pinned host memory and HBM:
+    `reps` copies of `gib` GiB host -> device, then device -> host, then both
+    directions at once (two streams, two host threads)."""
+    import ctypes
+    import threading
+
+    from regridding_amd._lib import DeviceArray, check, lib
+
+    L = lib()
+    n = gib << 30
+    dev_a = DeviceArray(n, np.uint8, (n,))
+    dev_b = DeviceArray(n, np.uint8, (n,))
+    src = ring.ctypes.data
+    dst = ring.ctypes.data + 2 * n
+    streams = [ctypes.c_void_p(), ctypes.c_void_p()]
+    for st in streams:
+        check(L.rg_stream_create(ctypes.byref(st)))
+
+    def h2d():
+        for _ in range(reps):
+            check(L.rg_memcpy_htod(dev_a.ptr, src, n, streams[0]))
+        check(L.rg_stream_sync(streams[0]))
+
+    def d2h():
+        for _ in range(reps):
+            check(L.rg_memcpy_dtoh(dst, dev_b.ptr, n, streams[1]))
+
+    out = {}
+    for name, fns in (("h2d", [h2d]), ("d2h", [d2h]), ("both", [h2d, d2h])):
+        fns[0]() if name != "both" else None  # warm
+        t0 = time.perf_counter()
+        th = [threading.Thread(target=f) for f in fns]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        dt = time.perf_counter() - t0
+        out[f"{name}_GBps"] = round(len(fns) * reps * n / dt / 1e9, 2)
+    for st in streams:
+        check(L.rg_stream_destroy(st))
+    log(f"PCIe (pinned host memory): {out}")
     return out
 
 
