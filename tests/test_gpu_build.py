@@ -39,6 +39,12 @@ CASES = {
                                     workloads.warped_grid(150, 0.4, 0.0, 0.0, -0.9, 0.9)),
     "output_covers_input": lambda: (workloads.warped_grid(20, 0.1, 0.02, 0.02, -0.5, 0.5),
                                     workloads.warped_grid(30, 0.3, 0.0, 0.0, -1.5, 1.5)),
+    # input cells about twice the output cells: 64-cell groups of 1025 to
+    # 4096 records (the block-per-group reduction)
+    "mid_groups": lambda: (workloads.warped_grid(65, 0.25, 0.04, 0.03),
+                           workloads.warped_grid(120, 0.5, 0.0, 0.0, -0.9, 0.9)),
+    "mid_groups_large_cells": lambda: (workloads.warped_grid(65, 0.2, 0.05, 0.02),
+                                       workloads.warped_grid(160, 0.45, 0.0, 0.0, -0.95, 0.95)),
 }
 
 
@@ -92,8 +98,9 @@ def test_build_fixup_rewalks_bad_guesses(case, K):
     assert h.stats()["n_fixups"] > 0
 
 
-def test_build_weights_input():
-    ci, co = _perturbed(*CASES["coarse_to_fine"]())
+@pytest.mark.parametrize("case", ["coarse_to_fine", "mid_groups"])
+def test_build_weights_input(case):
+    ci, co = _perturbed(*CASES[case]())
     wi = np.random.default_rng(7).uniform(0.5, 3.0, (ci[0].shape[0] - 1, ci[0].shape[1] - 1))
     got = build_2d(ci, co, wi).export()
     ref = oracle.weights_2d(ci, co, wi)
