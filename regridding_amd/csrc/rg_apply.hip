@@ -115,6 +115,81 @@ __device__ inline int32_t fast_div(int32_t a, int32_t b, float inv_b)
     return q;
 }
 
+// Stages the (rows x cols) box of nd images, image-major then row-major, into
+// patch (LDS).  With BY_COLUMN (the forward apply), a box of at least
+// APPLY_BLOCK values per image and at most APPLY_BLOCK wide is staged with
+// each thread owning one column of a band of APPLY_BLOCK / cols whole box
+// rows: the thread steps down the box rows of all nd images with
+// incremental offsets (one step = a fixed number of images and rows, one
+// conditional wrap), which costs a few integer operations per value where
+// the flattened index costs two divisions (the FP32 forward apply was bound
+// by that index arithmetic).  The adjoint keeps the flattened index (its
+// own instantiation), measured faster there.  stage_unroll<T>() values per thread are
+// loaded before they are stored.
+template <class T, bool BY_COLUMN>
+__device__ inline void stage_box(T* __restrict__ patch, const T* __restrict__ xb,
+                                 int64_t ncols, int64_t src_gy, int32_t rows, int32_t cols,
+                                 int32_t nd)
+{
+    constexpr int SU = stage_unroll<T>();
+    const int32_t t = threadIdx.x;
+    const int32_t nvr = rows * nd;  // box rows over all nd images
+    if (BY_COLUMN && cols <= APPLY_BLOCK && rows * cols >= APPLY_BLOCK) {
+        const int32_t rps = APPLY_BLOCK / cols;  // box rows per step (>= 1, <= rows)
+        const int32_t r0 = t / cols;
+        if (r0 >= rps) return;  // past the last whole row of the band
+        const int32_t jj = t - r0 * cols;
+        // one step of rps box rows = mstep images and rstep rows
+        const int32_t mstep = rps / rows, rstep = rps - mstep * rows;
+        const int64_t dstep = (int64_t)mstep * ncols + (int64_t)rstep * src_gy;
+        const int64_t dwrap = ncols - (int64_t)rows * src_gy;  // a row past the last: next image
+        const int32_t m0 = r0 / rows;
+        int32_t iq = r0 - m0 * rows;
+        int64_t g = (int64_t)m0 * ncols + (int64_t)iq * src_gy + jj;  // offset in xb of box row r0
+        const int32_t lstep = rps * cols;
+        int32_t l = r0 * cols + jj;
+        for (int32_t R = r0; R < nvr; R += rps * SU) {
+            T v[SU];
+#pragma unroll
+            for (int u = 0; u < SU; u++) {
+                if (R + u * rps < nvr) v[u] = xb[g];
+                g += dstep;
+                iq += rstep;
+                if (iq >= rows) {
+                    iq -= rows;
+                    g += dwrap;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < SU; u++)
+                if (R + u * rps < nvr) patch[l + u * lstep] = v[u];
+            l += SU * lstep;
+        }
+        return;
+    }
+    const int32_t ncell = rows * cols, total = ncell * nd;
+    const float inv_cell = 1.0f / (float)ncell, inv_cols = 1.0f / (float)cols;
+    for (int32_t e0 = t; e0 < total; e0 += APPLY_BLOCK * SU) {
+        T v[SU];
+#pragma unroll
+        for (int u = 0; u < SU; u++) {
+            const int32_t e = e0 + u * APPLY_BLOCK;
+            if (e < total) {
+                const int32_t m = fast_div(e, ncell, inv_cell);
+                const int32_t rem = e - m * ncell;
+                const int32_t ii = fast_div(rem, cols, inv_cols);
+                const int32_t jj = rem - ii * cols;
+                v[u] = xb[(int64_t)m * ncols + (int64_t)ii * src_gy + jj];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < SU; u++) {
+            const int32_t e = e0 + u * APPLY_BLOCK;
+            if (e < total) patch[e] = v[u];
+        }
+    }
+}
+
 __global__ __launch_bounds__(APPLY_BLOCK) void k_plan(
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
     ApplyGrid G, int64_t src_gy, int32_t* __restrict__ box,
@@ -188,7 +263,8 @@ __global__ void k_plan_max(const int32_t* __restrict__ box, int64_t ntiles,
 
 // lds_bytes: the dynamic LDS of the launch; a tile whose patch for the
 // block's images does not fit gathers from global memory instead.
-template <class T, int DB>
+// BY_COLUMN selects the staging index (stage_box).
+template <class T, int DB, bool BY_COLUMN>
 __global__ __launch_bounds__(APPLY_BLOCK) void k_apply_planned(
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
     const uint16_t* __restrict__ off, const int32_t* __restrict__ box,
@@ -209,33 +285,8 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_apply_planned(
     const int64_t nd = batch - d0 < DB ? batch - d0 : DB;
     const bool staged = cells > 0 && cells * nd * (int64_t)sizeof(T) <= lds_bytes;
     if (staged) {
-        // The box of nd images, flattened (image, row, column) over the
-        // block's lanes; each lane issues stage_unroll<T>() loads before its LDS
-        // stores, so the staging costs about one memory latency instead of
-        // one per box row.
-        const int32_t ncell = (int32_t)cells, total = ncell * (int32_t)nd;
-        const float inv_cell = 1.0f / (float)ncell, inv_cols = 1.0f / (float)cols;
-        const T* xb = x + d0 * ncols + (int64_t)i0 * src_gy + j0;
-        constexpr int SU = stage_unroll<T>();
-        for (int32_t e0 = threadIdx.x; e0 < total; e0 += APPLY_BLOCK * SU) {
-            T v[SU];
-#pragma unroll
-            for (int u = 0; u < SU; u++) {
-                const int32_t e = e0 + u * APPLY_BLOCK;
-                if (e < total) {
-                    const int32_t m = fast_div(e, ncell, inv_cell);
-                    const int32_t rem = e - m * ncell;
-                    const int32_t ii = fast_div(rem, cols, inv_cols);
-                    const int32_t jj = rem - ii * cols;
-                    v[u] = xb[(int64_t)m * ncols + (int64_t)ii * src_gy + jj];
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < SU; u++) {
-                const int32_t e = e0 + u * APPLY_BLOCK;
-                if (e < total) patch[e] = v[u];
-            }
-        }
+        stage_box<T, BY_COLUMN>(patch, x + d0 * ncols + (int64_t)i0 * src_gy + j0, ncols, src_gy,
+                                rows, cols, (int32_t)nd);
         __syncthreads();
     }
     const int32_t ti = (int32_t)(tile / G.ntx), tj = (int32_t)(tile - (int64_t)ti * G.ntx);
@@ -295,7 +346,7 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_apply_planned(
 // images the block stages the input patch in LDS and gathers from it.  The
 // entries and row pointers are read once per launch instead of once per
 // image group, and the blocks (one per tile) stay resident longer.
-template <class T, int DB, int MAXE>
+template <class T, int DB, int MAXE, bool BY_COLUMN>
 __global__ __launch_bounds__(APPLY_BLOCK) void k_apply_rounds(
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
     const uint16_t* __restrict__ off, const int32_t* __restrict__ box,
@@ -334,33 +385,11 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_apply_rounds(
         }
     }
     const bool fits = ncell > 0 && (int64_t)ncell * DB * (int64_t)sizeof(T) <= lds_bytes;
-    const float inv_cell = 1.0f / (float)(ncell > 0 ? ncell : 1);
-    const float inv_cols = 1.0f / (float)(cols > 0 ? cols : 1);
     for (int64_t d0 = 0; d0 < batch; d0 += DB) {
         const int64_t nd = batch - d0 < DB ? batch - d0 : DB;
         if (fits) {
-            const int32_t total = ncell * (int32_t)nd;
-            const T* xb = x + d0 * ncols + (int64_t)i0 * src_gy + j0;
-            constexpr int SU = stage_unroll<T>();
-            for (int32_t e0 = threadIdx.x; e0 < total; e0 += APPLY_BLOCK * SU) {
-                T v[SU];
-#pragma unroll
-                for (int u = 0; u < SU; u++) {
-                    const int32_t ee = e0 + u * APPLY_BLOCK;
-                    if (ee < total) {
-                        const int32_t m = fast_div(ee, ncell, inv_cell);
-                        const int32_t rem = ee - m * ncell;
-                        const int32_t ii = fast_div(rem, cols, inv_cols);
-                        const int32_t jj = rem - ii * cols;
-                        v[u] = xb[(int64_t)m * ncols + (int64_t)ii * src_gy + jj];
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < SU; u++) {
-                    const int32_t ee = e0 + u * APPLY_BLOCK;
-                    if (ee < total) patch[ee] = v[u];
-                }
-            }
+            stage_box<T, BY_COLUMN>(patch, x + d0 * ncols + (int64_t)i0 * src_gy + j0, ncols,
+                                    src_gy, rows, cols, (int32_t)nd);
             __syncthreads();
         }
         if (valid) {
@@ -416,12 +445,19 @@ __global__ void k_to_f32(const double* __restrict__ a, float* __restrict__ b,
 // blocks of APPLY_BLOCK output rows x DB images; block b finds its job in the
 // jobs' block prefix.  The accumulation per output is the one of k_apply.
 template <class T>
+//
+// Blocks are renumbered so each of the 8 XCDs takes one contiguous range of
+// the blocks (blocks arrive round-robin over the XCDs): neighbouring row
+// blocks of a job gather overlapping inputs, which then stay in one XCD's L2
+// instead of being fetched by all eight.
 __global__ __launch_bounds__(APPLY_BLOCK) void k_apply_jobs(
     const JobDesc* __restrict__ jobs, const int64_t* __restrict__ block_prefix,
-    int njobs, const int32_t* __restrict__ abort_flag)
+    int njobs, int64_t nblocks, const int32_t* __restrict__ abort_flag)
 {
     if (abort_flag && *abort_flag) return;  // uniform: one word for the grid
-    const int64_t b = blockIdx.x;
+    const int64_t ng = gridDim.x;  // a multiple of NUM_XCD
+    const int64_t b = (blockIdx.x % NUM_XCD) * (ng / NUM_XCD) + blockIdx.x / NUM_XCD;
+    if (b >= nblocks) return;
     const JobDesc& J = jobs[seg_of(block_prefix, njobs, b)];
     const int64_t lt = b - J.block_base;
     const int64_t grp = lt / J.nrt, rt = lt - grp * J.nrt;
@@ -548,10 +584,15 @@ int launch_apply(rg_weights* W, int dir, const int32_t* rowptr,
         lds = (lds + 255) / 256 * 256;
         if (lds > LDS_BYTES) lds = LDS_BYTES;
         dim3 g1((unsigned)nb);
-#define RG_ROUNDS(D)                                                           \
-    k_apply_rounds<T, D, 8><<<g1, bl, (size_t)lds, s>>>(rowptr, col, P->off, P->box, \
-                                                       w, x, y, nrows, ncols, src_gy, \
-                                                       batch, G, lds)
+#define RG_ROUNDS(D)                                                                   \
+    do {                                                                               \
+        if (dir == 0)                                                                  \
+            k_apply_rounds<T, D, 8, true><<<g1, bl, (size_t)lds, s>>>(                 \
+                rowptr, col, P->off, P->box, w, x, y, nrows, ncols, src_gy, batch, G, lds); \
+        else                                                                           \
+            k_apply_rounds<T, D, 8, false><<<g1, bl, (size_t)lds, s>>>(                \
+                rowptr, col, P->off, P->box, w, x, y, nrows, ncols, src_gy, batch, G, lds); \
+    } while (0)
         switch (db) {
         case 16: RG_ROUNDS(16); break;
         case 8: RG_ROUNDS(8); break;
@@ -564,10 +605,15 @@ int launch_apply(rg_weights* W, int dir, const int32_t* rowptr,
         int64_t lds = P->max_cells * db * (int64_t)sizeof(T);
         lds = (lds + 255) / 256 * 256;
         if (lds > LDS_BYTES) lds = LDS_BYTES;
-#define RG_PLANNED(D)                                                          \
-    k_apply_planned<T, D><<<g, bl, (size_t)lds, s>>>(rowptr, col, P->off, P->box, \
-                                                     w, x, y, nrows, ncols, src_gy, \
-                                                     batch, G, lds)
+#define RG_PLANNED(D)                                                                  \
+    do {                                                                               \
+        if (dir == 0)                                                                  \
+            k_apply_planned<T, D, true><<<g, bl, (size_t)lds, s>>>(                    \
+                rowptr, col, P->off, P->box, w, x, y, nrows, ncols, src_gy, batch, G, lds); \
+        else                                                                           \
+            k_apply_planned<T, D, false><<<g, bl, (size_t)lds, s>>>(                   \
+                rowptr, col, P->off, P->box, w, x, y, nrows, ncols, src_gy, batch, G, lds); \
+    } while (0)
         switch (db) {
         case 16: RG_PLANNED(16); break;
         case 8: RG_PLANNED(8); break;
@@ -718,12 +764,13 @@ int launch_jobs_common(const rg_apply_job* jobs, int64_t njobs, bool f32,
     up.recorded = true;
     const JobDesc* d_jobs = (const JobDesc*)dt.p;
     const int64_t* d_prefix = (const int64_t*)(dt.p + poff);
+    const unsigned grid = (unsigned)((nblocks + NUM_XCD - 1) / NUM_XCD * NUM_XCD);
     if (f32)
-        k_apply_jobs<float><<<(unsigned)nblocks, APPLY_BLOCK, 0, s>>>(
-            d_jobs, d_prefix, (int)table.size(), nullptr);
+        k_apply_jobs<float><<<grid, APPLY_BLOCK, 0, s>>>(d_jobs, d_prefix, (int)table.size(),
+                                                         nblocks, nullptr);
     else
-        k_apply_jobs<double><<<(unsigned)nblocks, APPLY_BLOCK, 0, s>>>(
-            d_jobs, d_prefix, (int)table.size(), nullptr);
+        k_apply_jobs<double><<<grid, APPLY_BLOCK, 0, s>>>(d_jobs, d_prefix, (int)table.size(),
+                                                          nblocks, nullptr);
     RG_HIP(hipGetLastError());
     for (int64_t k = 0; k < njobs; k++) RG_TRY(note_use(jobs[k].weights, s));
     return RG_OK;  // dt returns to the stream's pool after the kernel
@@ -736,8 +783,8 @@ int apply_jobs_device(const JobDesc* jobs, const int64_t* prefix, int njobs, int
 {
     if (njobs == 0 || nblocks == 0) return RG_OK;
     if (nblocks >= INT32_MAX) return fail(RG_ERR_ARGUMENT, "batched apply too large");
-    k_apply_jobs<double><<<(unsigned)nblocks, APPLY_BLOCK, 0, s>>>(jobs, prefix, njobs,
-                                                                  abort_flag);
+    const unsigned grid = (unsigned)((nblocks + NUM_XCD - 1) / NUM_XCD * NUM_XCD);
+    k_apply_jobs<double><<<grid, APPLY_BLOCK, 0, s>>>(jobs, prefix, njobs, nblocks, abort_flag);
     RG_HIP(hipGetLastError());
     return RG_OK;
 }
