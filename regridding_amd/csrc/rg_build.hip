@@ -524,6 +524,15 @@ __device__ inline const Build& build_of_thread(const Build* __restrict__ builds,
     return builds[__builtin_amdgcn_readfirstlane(pair_of_wave[t >> 6])];
 }
 
+// The pass of a walk thread's chunk, read as a wave-uniform value: each pass
+// (and each pair) starts its threads at a multiple of 64, so the active lanes
+// of a wavefront share one pass, and the pass table's fields then go through
+// the scalar cache into scalar registers.
+__device__ inline const Pass& pass_of_wave(const Build& B, const ChunkRef& r)
+{
+    return B.pass[__builtin_amdgcn_readfirstlane(r.p)];
+}
+
 __global__ __launch_bounds__(WALK_MAX_BLOCK) void k_chunk_start(const Build* __restrict__ builds,
                               const int32_t* __restrict__ pair_of_wave,
                               int64_t nthreads_all, int32_t* __restrict__ start,
@@ -535,7 +544,7 @@ __global__ __launch_bounds__(WALK_MAX_BLOCK) void k_chunk_start(const Build* __r
     const ChunkRef r = chunk_of_thread(B, t - B.thread_gbase);
     if (!r.valid || r.c == 0) return;
     start[B.chunk_gbase + r.g] =
-        guess_outside ? STATE_OUTSIDE : chunk_start_guess(B.pass[r.p], r.row, r.c * B.K);
+        guess_outside ? STATE_OUTSIDE : chunk_start_guess(pass_of_wave(B, r), r.row, r.c * B.K);
 }
 
 // ---- walks ---------------------------------------------------------------
@@ -563,7 +572,7 @@ __global__ __launch_bounds__(WALK_MAX_BLOCK) void k_walk_count(const Build* __re
     const Build& B = build_of_thread(builds, pair_of_wave, t);
     const ChunkRef r = chunk_of_thread(B, t - B.thread_gbase);
     if (!r.valid) return;
-    const Pass& P = B.pass[r.p];
+    const Pass& P = pass_of_wave(B, r);
     const int64_t g = B.chunk_gbase + r.g;
     int64_t k0, k1;
     chunk_range(B, P, r.c, &k0, &k1);
@@ -664,7 +673,7 @@ __global__ __launch_bounds__(WALK_MAX_BLOCK) void k_restage(const Build* __restr
     if (!r.valid) return;
     const int64_t g = B.chunk_gbase + r.g;
     if (count[g] <= cap) return;
-    const Pass& P = B.pass[r.p];
+    const Pass& P = pass_of_wave(B, r);
     int64_t k0, k1;
     chunk_range(B, P, r.c, &k0, &k1);
     // (a capacity-mode build sizes the buffer in advance: a chunk past its

@@ -15,6 +15,23 @@
 
 namespace rg {
 
+// Element i of an array in device global memory.  A pointer read from a
+// table in device memory (the grids of a Build) carries no address space,
+// so plain loads through it are flat loads, whose completion is also counted
+// on the LDS / scalar-load counter: every wait for one then waits for both
+// kinds.  On the device the load names the global address space; the host
+// twin serves the CPU emulation of the same functions.
+template <class T, class I>
+__device__ __forceinline__ T gload(const T* p, I i)
+{
+    return ((const __attribute__((address_space(1))) T*)p)[i];
+}
+template <class T, class I>
+__host__ __forceinline__ T gload(const T* p, I i)
+{
+    return p[i];
+}
+
 // geometry.py:153-284 bounding_boxes_intersect_2d: inclusive (touching
 // boxes intersect); each box is given by two corner points in any order.
 RG_HD bool bbox_overlap(double xp1, double yp1, double xp2, double yp2,
@@ -228,10 +245,10 @@ RG_HD double2 bilinear_xy(const double2* __restrict__ a, int64_t nx, int64_t ny,
     else if (x0 > nx - 2) x0 = nx - 2;
     if (y0 < 0) y0 = 0;
     else if (y0 > ny - 2) y0 = ny - 2;
-    const double2 a00 = a[x0 * ny + y0];
-    const double2 a01 = a[x0 * ny + y0 + 1];
-    const double2 a10 = a[(x0 + 1) * ny + y0];
-    const double2 a11 = a[(x0 + 1) * ny + y0 + 1];
+    const double2 a00 = gload(a, x0 * ny + y0);
+    const double2 a01 = gload(a, x0 * ny + y0 + 1);
+    const double2 a10 = gload(a, (x0 + 1) * ny + y0);
+    const double2 a11 = gload(a, (x0 + 1) * ny + y0 + 1);
     double dx = x - (double)x0;
     double dy = y - (double)y0;
     double w00 = (1 - dx) * (1 - dy);
@@ -248,7 +265,8 @@ RG_HD bool cell_contains_xy(const double2* __restrict__ xy, int64_t ny, int64_t 
                             double px, double py)
 {
     int64_t a = i * ny + j, b = (i + 1) * ny + j;
-    const double2 v0 = xy[a], v1 = xy[b], v2 = xy[b + 1], v3 = xy[a + 1];
+    const double2 v0 = gload(xy, a), v1 = gload(xy, b), v2 = gload(xy, b + 1),
+                  v3 = gload(xy, a + 1);
     double w = 0;
     w += winding_term(px, py, v3.x, v3.y, v0.x, v0.y);
     w += winding_term(px, py, v0.x, v0.y, v1.x, v1.y);

@@ -191,12 +191,12 @@ RG_HD int pass_of_row(const Build& B, int64_t r)
 
 RG_HD double sweep_x(const Pass& P, int64_t row, int64_t k)
 {
-    return P.xs[row * P.sr + k * P.sk];
+    return gload(P.xs, row * P.sr + k * P.sk);
 }
 
 RG_HD double sweep_y(const Pass& P, int64_t row, int64_t k)
 {
-    return P.ys[row * P.sr + k * P.sk];
+    return gload(P.ys, row * P.sr + k * P.sk);
 }
 
 // Scan-order code of boundary edge e: 2 * face + axis, the value the
@@ -265,20 +265,24 @@ RG_HD bool outside_scan(const StaticGrid& s, double x1, double y1, double x2,
     const double xl = x1 > x2 ? x2 : x1, xh = x1 > x2 ? x1 : x2;
     const double yl = y1 > y2 ? y2 : y1, yh = y1 > y2 ? y1 : y2;
     const int32_t nsup = (int32_t)s.nsup, nblk = (int32_t)s.nblk, nbe = (int32_t)s.nbe;
+    const Box* const bsup = s.bsup;
+    const Box* const bblk = s.bblk;
+    const double2* const be = (const double2*)s.be;  // 2 per edge
+    const int32_t* const bcell = s.bcell;
     double t_min = __builtin_huge_val();
     int32_t best = -1;
-    if (!box_touch(*s.ball, xl, yl, xh, yh)) return false;
+    if (!box_touch(gload(s.ball, 0), xl, yl, xh, yh)) return false;
     for (int32_t su = 0; su < nsup; su++) {
-        if (!box_touch(s.bsup[su], xl, yl, xh, yh)) continue;
+        if (!box_touch(gload(bsup, su), xl, yl, xh, yh)) continue;
         const int32_t b1 = min((su + 1) * BND_SUPER, nblk);
         for (int32_t bl = su * BND_SUPER; bl < b1; bl++) {
-            if (!box_touch(s.bblk[bl], xl, yl, xh, yh)) continue;
+            if (!box_touch(gload(bblk, bl), xl, yl, xh, yh)) continue;
             const int32_t e1 = min((bl + 1) * BND_BLOCK, nbe);
             for (int32_t e = bl * BND_BLOCK; e < e1; e++) {
-                const double* q = s.be + 4 * e;
-                TU r = seg_params(x1, y1, x2, y2, q[0], q[1], q[2], q[3]);
+                const double2 qa = gload(be, 2 * e), qb = gload(be, 2 * e + 1);
+                TU r = seg_params(x1, y1, x2, y2, qa.x, qa.y, qb.x, qb.y);
                 if (seg_hit(r) && r.t < t_min) {
-                    if (s.bcell[e] == skip_cell) continue;
+                    if (gload(bcell, e) == skip_cell) continue;
                     t_min = r.t;
                     best = e;
                 }
@@ -286,7 +290,7 @@ RG_HD bool outside_scan(const StaticGrid& s, double x1, double y1, double x2,
         }
     }
     if (best >= 0) {
-        *cell = s.bcell[best];
+        *cell = gload(bcell, best);
         *code = boundary_code(s, best);
         *t_hit = t_min;
     }
@@ -333,37 +337,51 @@ RG_HD int32_t walk_chunk(const Build& B, const Pass& P, int64_t row64,
 {
     // Indices are 32-bit: a build requires fewer than 2^31 vertices per grid
     // (so every vertex, cell and sweep index below fits).
+    // Everything the steps read from the pass is copied to locals first: the
+    // stores and atomics of the sink could alias the pass table as far as the
+    // compiler knows, and it would otherwise reload those fields (and wait
+    // for them) in every step.
     const StaticGrid& s = P.st;
+    const double2* const sxy = s.xy;
+    const double2* const wxy = P.xys;
+    const bool axis0 = P.axis == 0;
+    const bool sweep_in = P.sweep_input != 0;
     const int32_t ncx = (int32_t)(s.nx - 1), ncy = (int32_t)(s.ny - 1), nys = (int32_t)s.ny;
     const int32_t row = (int32_t)row64, kend = (int32_t)k1;
     const int32_t ncy_sweep = (int32_t)P.ncy_sweep, nrows = (int32_t)P.nrows;
     const int32_t sk = (int32_t)P.sk;
-    const int32_t limit = (int32_t)min(step_limit(s, k1 - k0), (int64_t)INT32_MAX - 1);
+    // steps left in the budget (one register, counted down)
+    int32_t budget = (int32_t)min(step_limit(s, k1 - k0), (int64_t)INT32_MAX - 1);
     int32_t k = (int32_t)k0;
     int32_t widx = row * (int32_t)P.sr + k * sk;  // sweep vertex (row, k)
-    const double2 sv0 = P.xys[widx];
+    const double2 sv0 = gload(wxy, widx);
     double x1 = sv0.x, y1 = sv0.y;
     // the end of the current sweep edge, loaded when the walk reaches a
-    // new edge (a crossing keeps it)
+    // new edge (a crossing keeps it).  (Requesting the next edge's end one
+    // step ahead costs registers -- a wavefront per SIMD -- and measured
+    // slower.)
     double2 ev = make_double2(0, 0);
-    if (k < kend) ev = P.xys[widx + sk];
+    if (k < kend) ev = gload(wxy, widx + sk);
     bool outside = start < 0;
     int32_t sx = outside ? 0 : start / ncy;
     int32_t sy = outside ? 0 : start % ncy;
     int32_t edge_last = -1;  // sys.maxsize
-    int32_t steps = 0;
-    int32_t csx = -2, csy = -2;  // cell whose corners are cached
+    // How the walk reached the current cell, i.e. which cached corners still
+    // hold: the edge crossed out of the previous cell (0..3: the previous
+    // cell's corners shift over and two are loaded), -1 (same cell: all
+    // four hold) or -2 (none: all four are loaded).
+    int32_t mv = -2;
     double2 c0, c1, c2, c3;
     c0.x = c0.y = c1.x = c1.y = c2.x = c2.y = c3.x = c3.y = 0;
     while (k < kend) {
-        if (++steps > limit) return STATE_FAILED;
+        if (--budget < 0) return STATE_FAILED;
         double x2 = ev.x, y2 = ev.y;
         if (outside) {
             int32_t cell;
             int code;
             double t;
             // the cached corners are reloaded after an outside stretch
-            csx = csy = -2;
+            mv = -2;
             c0.x = c0.y = c1.x = c1.y = c2.x = c2.y = c3.x = c3.y = 0;
             if (outside_scan(s, x1, y1, x2, y2, edge_last, &cell, &code, &t)) {
                 x2 = x1 + t * (x2 - x1);
@@ -376,7 +394,7 @@ RG_HD int32_t walk_chunk(const Build& B, const Pass& P, int64_t row64,
                 k += 1;
                 widx += sk;
                 edge_last = -1;
-                if (k < kend) ev = P.xys[widx + sk];
+                if (k < kend) ev = gload(wxy, widx + sk);
             }
         } else {
             // _step_inside_static (:561-749).  Cell corners c0..c3 =
@@ -384,21 +402,20 @@ RG_HD int32_t walk_chunk(const Build& B, const Pass& P, int64_t row64,
             // outward neighbour is cell_normals[v].  The corners stay in
             // registers while the walk stays in the cell; a move to a
             // neighbour loads only the two new corners.
-            if (sx != csx || sy != csy) {
+            if (mv != -1) {
                 const int32_t a = sx * nys + sy, b = a + nys;
-                if (sx == csx + 1 && sy == csy) {
-                    c0 = c1; c3 = c2; c1 = s.xy[b]; c2 = s.xy[b + 1];
-                } else if (sx == csx - 1 && sy == csy) {
-                    c1 = c0; c2 = c3; c0 = s.xy[a]; c3 = s.xy[a + 1];
-                } else if (sy == csy + 1 && sx == csx) {
-                    c0 = c3; c1 = c2; c3 = s.xy[a + 1]; c2 = s.xy[b + 1];
-                } else if (sy == csy - 1 && sx == csx) {
-                    c3 = c0; c2 = c1; c0 = s.xy[a]; c1 = s.xy[b];
+                if (mv == 2) {         // moved +x
+                    c0 = c1; c3 = c2; c1 = gload(sxy, b); c2 = gload(sxy, b + 1);
+                } else if (mv == 0) {  // -x
+                    c1 = c0; c2 = c3; c0 = gload(sxy, a); c3 = gload(sxy, a + 1);
+                } else if (mv == 3) {  // +y
+                    c0 = c3; c1 = c2; c3 = gload(sxy, a + 1); c2 = gload(sxy, b + 1);
+                } else if (mv == 1) {  // -y
+                    c3 = c0; c2 = c1; c0 = gload(sxy, a); c1 = gload(sxy, b);
                 } else {
-                    c0 = s.xy[a]; c1 = s.xy[b]; c2 = s.xy[b + 1]; c3 = s.xy[a + 1];
+                    c0 = gload(sxy, a); c1 = gload(sxy, b); c2 = gload(sxy, b + 1);
+                    c3 = gload(sxy, a + 1);
                 }
-                csx = sx;
-                csy = sy;
             }
             const double cx0 = c0.x, cy0 = c0.y, cx1 = c1.x, cy1 = c1.y;
             const double cx2 = c2.x, cy2 = c2.y, cx3 = c3.x, cy3 = c3.y;
@@ -436,15 +453,15 @@ RG_HD int32_t walk_chunk(const Build& B, const Pass& P, int64_t row64,
             // _calc_and_save_weights (:757-830) for p1 -> p2 at sweep
             // index (row, k) and static cell (sx, sy).
             double area = area_tri(x1, y1, x2, y2);
-            if (P.axis == 0) area = -area;
+            if (axis0) area = -area;
             const int32_t cell_static = sx * ncy + sy;
             for (int side = 0; side < 2; side++) {
                 const int32_t rr = side == 0 ? row - 1 : row;
                 if (side == 0 && !(rr >= 0)) continue;
                 if (side == 1 && !(rr < nrows - 1)) continue;
-                const int32_t cell_sweep = P.axis == 0 ? k * ncy_sweep + rr : rr * ncy_sweep + k;
-                const int32_t cin = P.sweep_input ? cell_sweep : cell_static;
-                const int32_t cout = P.sweep_input ? cell_static : cell_sweep;
+                const int32_t cell_sweep = axis0 ? k * ncy_sweep + rr : rr * ncy_sweep + k;
+                const int32_t cin = sweep_in ? cell_sweep : cell_static;
+                const int32_t cout = sweep_in ? cell_static : cell_sweep;
                 // the weight is term / area(cin) [* weights_input[cin]]
                 // (:797-802, :820-825); the reduction divides
                 sink.emit(cin, cout, side == 0 ? area : -area);
@@ -456,9 +473,10 @@ RG_HD int32_t walk_chunk(const Build& B, const Pass& P, int64_t row64,
                 sx = nsx;
                 sy = nsy;
             }
+            mv = hit;  // (-1: still in the same cell)
             if (k_next != k) {
                 widx += sk;
-                if (k_next < kend) ev = P.xys[widx + sk];
+                if (k_next < kend) ev = gload(wxy, widx + sk);
             }
             k = k_next;
             edge_last = e_next;
@@ -477,8 +495,9 @@ RG_HD int32_t chunk_start_guess(const Pass& P, int64_t row, int64_t k)
 {
     const StaticGrid& s = P.st;
     double px = sweep_x(P, row, k), py = sweep_y(P, row, k);
-    if (!(s.bbox[0] <= px && px <= s.bbox[2] && s.bbox[1] <= py &&
-          py <= s.bbox[3]))
+    const double bx0 = gload(s.bbox, 0), by0 = gload(s.bbox, 1);
+    const double bx1 = gload(s.bbox, 2), by1 = gload(s.bbox, 3);
+    if (!(bx0 <= px && px <= bx1 && by0 <= py && py <= by1))
         return STATE_OUTSIDE;
     int64_t c = index_secant_xy(s.xy, s.nx, s.ny, px, py);
     return c >= 0 ? (int32_t)c : STATE_OUTSIDE;
