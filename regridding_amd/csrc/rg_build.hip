@@ -331,7 +331,7 @@ __global__ void k_grid_poly_yranges(const GridDesc* __restrict__ grids,
                                     double2* __restrict__ pyr)
 {
     // one wavefront per run, one lane per edge
-    const int64_t b = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t b = grid_wave();
     const int lane = threadIdx.x & 63;
     if (b >= npyr_all) return;  // uniform per wavefront
     const GridDesc& G = grids[seg_of(pyr_prefix, ngrids, b)];
@@ -457,7 +457,7 @@ __global__ void k_row_start(const Build* __restrict__ builds,
                             int64_t nrows_all, int32_t* __restrict__ start,
                             int32_t* __restrict__ anomalies)
 {
-    const int64_t gw = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t gw = grid_wave();
     const int lane = threadIdx.x & 63;
     if (gw >= nrows_all) return;  // uniform per wavefront
     const Build& B = builds[__builtin_amdgcn_readfirstlane(pair_of_row[gw])];
@@ -595,7 +595,7 @@ __global__ __launch_bounds__(WALK_MAX_BLOCK) void k_fixup(const Build* __restric
                         int32_t* __restrict__ failed,
                         Triple* __restrict__ stage, int64_t cap)
 {
-    const int64_t gr = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t gr = grid_wave();
     const int lane = threadIdx.x & 63;
     if (gr >= nrows_all) return;  // uniform per wavefront
     const Build& B = builds[pair_of_row[gr]];
@@ -964,9 +964,11 @@ __device__ inline void put_reduced_cell(KV* __restrict__ dst, const uint64_t* K,
     }
 }
 
-// The records of one group, reached through its four pass ranges.
+// The records of one group, reached through its four pass ranges: their
+// total (the group's count, of which group_off is the exclusive scan) and
+// the end of the last range, which ends last (the records sit pass-major).
 struct GroupRecs {
-    int64_t ps0, ps1, ps2, ps3, e0, e1, e2;
+    int64_t ps0, ps1, ps2, ps3, e0, e1, e2, total, end;
     __device__ GroupRecs(const int64_t* __restrict__ bucket_off, int64_t n_groups, int64_t grp)
     {
         ps0 = bucket_off[grp];
@@ -976,6 +978,8 @@ struct GroupRecs {
         e0 = bucket_off[grp + 1] - ps0;
         e1 = e0 + bucket_off[n_groups + grp + 1] - ps1;
         e2 = e1 + bucket_off[2 * n_groups + grp + 1] - ps2;
+        end = bucket_off[3 * n_groups + grp + 1];
+        total = e2 + end - ps3;
     }
     // position of record q of the concatenated pass ranges
     __device__ int64_t src(int64_t q) const
@@ -1002,14 +1006,22 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
     __shared__ uint64_t sk[RED_WAVES][RED_CAP];
     __shared__ double sv[RED_WAVES][RED_CAP];
     __shared__ int32_t scnt[RED_WAVES][64];
-    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wid = block_wave(), lane = threadIdx.x & 63;
     if (blockIdx.x == 0 && threadIdx.x == 0) cell_off[n_groups * 64] = group_off[n_groups];
     const int64_t grp = (int64_t)blockIdx.x * RED_WAVES + wid;
     if (grp >= n_groups) return;  // uniform per wavefront; no block barriers below
+    // requested first, with the group's offsets: the cell areas and the
+    // pair's output-cell base, which the records do not decide
+    const double vol = vol_in[grp * 64 + lane];
+    int32_t* const hist_out =
+        hist_out_all + pcs[__builtin_amdgcn_readfirstlane(pair_of_group[grp])].out_gbase;
+    // every offset of the group takes part in the first test, so they are
+    // requested together (a load first used after a test is not issued
+    // before it: one round trip per test)
     const int64_t base = group_off[grp];
-    const int64_t total = group_off[grp + 1] - base;
-    // (the records sit pass-major: the group's last pass range ends last)
-    if (base + total > red_cap || bucket_off[3 * n_groups + grp + 1] > red_cap) {
+    const GroupRecs G(bucket_off, n_groups, grp);
+    const int64_t total = G.total;
+    if (base + total > red_cap || G.end > red_cap) {
         // past the records a capacity-mode build holds: nothing of it is kept
         nnz_row[grp * 64 + lane] = 0;
         cell_off[grp * 64 + lane] = 0;
@@ -1025,7 +1037,6 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
     }
     const int64_t i = grp * 64 + lane;
     int32_t* C = scnt[wid];
-    const GroupRecs G(bucket_off, n_groups, grp);
     // the records are read once, into registers (record q = lane + 64 u of
     // the concatenated pass ranges)
     KV rr[RED_CAP / 64];
@@ -1050,7 +1061,6 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
     const int32_t n = C[lane];
     const int32_t s = wave_exclusive(n, lane);
     cell_off[i] = base + s;
-    const double vol = vol_in[i];
     uint64_t* K = sk[wid];
     double* V = sv[wid];
     int32_t nmax = n;
@@ -1078,8 +1088,6 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
         bool lr1 = false;  // (a run of at most RED_NET terms is never long)
         const int64_t m1 = bucket_reduce_sorted(K + lane, V + lane, n, &lr1, 64);
         const int64_t dst1 = wave_exclusive((int32_t)m1, lane);
-        int32_t* hist_out =
-            hist_out_all + pcs[__builtin_amdgcn_readfirstlane(pair_of_group[grp])].out_gbase;
         put_reduced_cell(red + base + dst1, K + lane, V + lane, m1, hist_out, 64);
         nnz_row[i] = (int32_t)m1;
         if (lane == 0) group_mode[grp] = 0;
@@ -1134,8 +1142,6 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
             red[base + dst + t] = r;
         }
     } else {
-        int32_t* hist_out =
-            hist_out_all + pcs[__builtin_amdgcn_readfirstlane(pair_of_group[grp])].out_gbase;
         put_reduced_cell(red + base + dst, K + s, V + s, m, hist_out);
     }
     nnz_row[i] = (int32_t)m;
@@ -1180,7 +1186,7 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_heavy(
     __shared__ double sv[RED_WAVES][HEAVY_CAP];
     __shared__ uint8_t sc[RED_WAVES][HEAVY_CAP];
     __shared__ int32_t scnt[RED_WAVES][64];
-    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wid = block_wave(), lane = threadIdx.x & 63;
     int32_t* C = scnt[wid];
     uint64_t* K = sk[wid];
     double* V = sv[wid];
@@ -1191,13 +1197,16 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_heavy(
     for (;;) {
         int32_t h = 0;
         if (lane == 0) h = atomicAdd(heavy_next, 1);
-        h = __shfl(h, 0, 64);
+        // (all lanes are active here: the first active lane is lane 0, and
+        // the group's tables are then read with scalar loads)
+        h = __builtin_amdgcn_readfirstlane(h);
         if (h >= nh) break;  // uniform per wavefront
         const int64_t grp = heavy_list[h];
         const int64_t i = grp * 64 + lane;
+        const double vol = vol_in[i];
         const int64_t base = group_off[grp];
-        const int64_t total = group_off[grp + 1] - base;
         const GroupRecs G(bucket_off, n_groups, grp);
+        const int64_t total = G.total;
         int32_t* hist_out =
             hist_out_all + pcs[__builtin_amdgcn_readfirstlane(pair_of_group[grp])].out_gbase;
         C[lane] = 0;
@@ -1215,7 +1224,6 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_heavy(
         const int32_t n = C[lane];
         const int32_t s = wave_exclusive(n, lane);
         cell_off[i] = base + s;
-        const double vol = vol_in[i];
         wave_sync_lds();
         int c = 0;  // first cell of the next run
         while (c < 64) {
@@ -1384,7 +1392,7 @@ __global__ void k_csr_groups(const PairCells* __restrict__ pcs,
                              int32_t* __restrict__ t_col,
                              double* __restrict__ t_val, int64_t nnz_cap)
 {
-    const int64_t grp = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t grp = grid_wave();
     const int lane = threadIdx.x & 63;
     if (grp >= n_groups) return;  // uniform per wavefront
     const int k = __builtin_amdgcn_readfirstlane(pair_of_group[grp]);
@@ -1493,7 +1501,7 @@ __global__ __launch_bounds__(SORT_WAVES * 64) void k_rows_sort(
 {
     __shared__ int32_t sc[SORT_WAVES][SORT_CAP];
     __shared__ double sv[SORT_WAVES][SORT_CAP];
-    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wid = block_wave(), lane = threadIdx.x & 63;
     const int64_t r0 = ((int64_t)blockIdx.x * SORT_WAVES + wid) * 64;
     if (r0 >= nrows_all) return;  // uniform per wavefront
     const int k = __builtin_amdgcn_readfirstlane(pair_of_group[r0 >> 6]);

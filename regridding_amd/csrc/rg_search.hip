@@ -38,7 +38,7 @@ __global__ void k_cells_finish(const double* __restrict__ x,
                                const int64_t* __restrict__ cell,
                                int64_t* __restrict__ ij)
 {
-    const int64_t q = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t q = grid_wave();
     const int lane = threadIdx.x & 63;
     if (q >= n) return;
     int64_t c = cell[q];

@@ -5,6 +5,23 @@
 
 namespace rg {
 
+// The calling wavefront's index in its block, and in the grid (blocks of
+// whole wavefronts), as wave-uniform values.  The compiler cannot tell that
+// threadIdx.x >> 6 is the same on every lane; whatever a kernel derives from
+// it would sit in vector registers and be read with vector loads, one
+// dependent round trip after another.  Read this way it sits in a scalar
+// register, and the per-wavefront tables it indexes are read with scalar
+// loads.
+__device__ __forceinline__ int block_wave()
+{
+    return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+}
+
+__device__ __forceinline__ int64_t grid_wave()
+{
+    return (int64_t)blockIdx.x * (blockDim.x >> 6) + block_wave();
+}
+
 __device__ inline double wave_sum(double v)
 {
 #pragma unroll
