@@ -94,18 +94,11 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_apply(
 // row load per box row, and gathers from LDS.
 
 constexpr int LDS_BYTES = 64 * 1024;  // staged patch per block
-// Values each lane loads before its LDS stores in a staging step (measured:
-// more per lane gains nothing for either precision, fewer loses for FP64).
-template <class T>
-constexpr int stage_unroll()
-{
-    return 8;
-}
 constexpr int64_t PLAN_MIN_ROWS = 65536;
 
 // a / b from a float reciprocal and one correction step (the float quotient
 // is off by at most one).  Exact for 0 <= a <= 65536 and 1 <= b <= 16384,
-// checked exhaustively; a staged patch holds at most 16384 elements.
+// checked exhaustively; a staged patch holds at most 16384 cells.
 __device__ inline int32_t fast_div(int32_t a, int32_t b, float inv_b)
 {
     int32_t q = (int32_t)((float)a * inv_b);
@@ -115,77 +108,50 @@ __device__ inline int32_t fast_div(int32_t a, int32_t b, float inv_b)
     return q;
 }
 
-// Stages the (rows x cols) box of nd images, image-major then row-major, into
-// patch (LDS).  With BY_COLUMN (the forward apply), a box of at least
-// APPLY_BLOCK values per image and at most APPLY_BLOCK wide is staged with
-// each thread owning one column of a band of APPLY_BLOCK / cols whole box
-// rows: the thread steps down the box rows of all nd images with
-// incremental offsets (one step = a fixed number of images and rows, one
-// conditional wrap), which costs a few integer operations per value where
-// the flattened index costs two divisions (the FP32 forward apply was bound
-// by that index arithmetic).  The adjoint keeps the flattened index (its
-// own instantiation), measured faster there.  stage_unroll<T>() values per thread are
-// loaded before they are stored.
-template <class T, bool BY_COLUMN>
-__device__ inline void stage_box(T* __restrict__ patch, const T* __restrict__ xb,
-                                 int64_t ncols, int64_t src_gy, int32_t rows, int32_t cols,
-                                 int32_t nd)
+// One input cell of a staged patch: its values in the DB images of the pass,
+// side by side.  The patch is cell-major, so a gather reads one entry's DB
+// values with one or a few wide LDS loads at one address; an image-major
+// patch needs a load and an address computation per image (the FP32 apply
+// was bound by that arithmetic).  Images past the pass's count hold 0.
+template <class T, int DB>
+struct alignas(DB * sizeof(T) >= 16 ? 16 : DB * sizeof(T)) CellVals {
+    T v[DB];
+};
+
+// Cells each thread stages per step (their DB loads in flight together).
+template <class T, int DB>
+constexpr int stage_cells_per_step()
 {
-    constexpr int SU = stage_unroll<T>();
-    const int32_t t = threadIdx.x;
-    const int32_t nvr = rows * nd;  // box rows over all nd images
-    if (BY_COLUMN && cols <= APPLY_BLOCK && rows * cols >= APPLY_BLOCK) {
-        const int32_t rps = APPLY_BLOCK / cols;  // box rows per step (>= 1, <= rows)
-        const int32_t r0 = t / cols;
-        if (r0 >= rps) return;  // past the last whole row of the band
-        const int32_t jj = t - r0 * cols;
-        // one step of rps box rows = mstep images and rstep rows
-        const int32_t mstep = rps / rows, rstep = rps - mstep * rows;
-        const int64_t dstep = (int64_t)mstep * ncols + (int64_t)rstep * src_gy;
-        const int64_t dwrap = ncols - (int64_t)rows * src_gy;  // a row past the last: next image
-        const int32_t m0 = r0 / rows;
-        int32_t iq = r0 - m0 * rows;
-        int64_t g = (int64_t)m0 * ncols + (int64_t)iq * src_gy + jj;  // offset in xb of box row r0
-        const int32_t lstep = rps * cols;
-        int32_t l = r0 * cols + jj;
-        for (int32_t R = r0; R < nvr; R += rps * SU) {
-            T v[SU];
-#pragma unroll
-            for (int u = 0; u < SU; u++) {
-                if (R + u * rps < nvr) v[u] = xb[g];
-                g += dstep;
-                iq += rstep;
-                if (iq >= rows) {
-                    iq -= rows;
-                    g += dwrap;
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < SU; u++)
-                if (R + u * rps < nvr) patch[l + u * lstep] = v[u];
-            l += SU * lstep;
-        }
-        return;
-    }
-    const int32_t ncell = rows * cols, total = ncell * nd;
-    const float inv_cell = 1.0f / (float)ncell, inv_cols = 1.0f / (float)cols;
-    for (int32_t e0 = t; e0 < total; e0 += APPLY_BLOCK * SU) {
-        T v[SU];
+    return DB >= 16 ? 1 : 16 / DB;
+}
+
+// Stages the (rows x cols) box of nd images at xb (image stride ncols, row
+// stride src_gy) into `patch`, cell (ii, jj) at ii * cols + jj.  Consecutive
+// threads take consecutive cells, so each image's loads are coalesced.
+template <class T, int DB>
+__device__ inline void stage_cells(CellVals<T, DB>* __restrict__ patch,
+                                   const T* __restrict__ xb, int64_t ncols, int64_t src_gy,
+                                   int32_t rows, int32_t cols, int32_t nd)
+{
+    constexpr int SU = stage_cells_per_step<T, DB>();
+    const int32_t ncell = rows * cols;
+    const float inv_cols = 1.0f / (float)cols;
+    for (int32_t c0 = threadIdx.x; c0 < ncell; c0 += APPLY_BLOCK * SU) {
+        CellVals<T, DB> cv[SU];
 #pragma unroll
         for (int u = 0; u < SU; u++) {
-            const int32_t e = e0 + u * APPLY_BLOCK;
-            if (e < total) {
-                const int32_t m = fast_div(e, ncell, inv_cell);
-                const int32_t rem = e - m * ncell;
-                const int32_t ii = fast_div(rem, cols, inv_cols);
-                const int32_t jj = rem - ii * cols;
-                v[u] = xb[(int64_t)m * ncols + (int64_t)ii * src_gy + jj];
+            const int32_t c = c0 + u * APPLY_BLOCK;
+            if (c < ncell) {
+                const int32_t ii = fast_div(c, cols, inv_cols);
+                const T* xc = xb + (int64_t)ii * src_gy + (c - ii * cols);
+#pragma unroll
+                for (int m = 0; m < DB; m++) cv[u].v[m] = m < nd ? xc[m * ncols] : T(0);
             }
         }
 #pragma unroll
         for (int u = 0; u < SU; u++) {
-            const int32_t e = e0 + u * APPLY_BLOCK;
-            if (e < total) patch[e] = v[u];
+            const int32_t c = c0 + u * APPLY_BLOCK;
+            if (c < ncell) patch[c] = cv[u];
         }
     }
 }
@@ -263,8 +229,7 @@ __global__ void k_plan_max(const int32_t* __restrict__ box, int64_t ntiles,
 
 // lds_bytes: the dynamic LDS of the launch; a tile whose patch for the
 // block's images does not fit gathers from global memory instead.
-// BY_COLUMN selects the staging index (stage_box).
-template <class T, int DB, bool BY_COLUMN>
+template <class T, int DB>
 __global__ __launch_bounds__(APPLY_BLOCK) void k_apply_planned(
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
     const uint16_t* __restrict__ off, const int32_t* __restrict__ box,
@@ -273,7 +238,7 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_apply_planned(
     int64_t lds_bytes)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    T* patch = reinterpret_cast<T*>(lds_raw);
+    CellVals<T, DB>* patch = reinterpret_cast<CellVals<T, DB>*>(lds_raw);
     const int64_t nb = gridDim.x;  // a multiple of NUM_XCD
     const int64_t b = blockIdx.x;
     const int64_t tile = (b % NUM_XCD) * (nb / NUM_XCD) + b / NUM_XCD;
@@ -282,11 +247,11 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_apply_planned(
     const int32_t rows = box[4 * tile + 2], cols = box[4 * tile + 3];
     const int64_t cells = (int64_t)rows * cols;
     const int64_t d0 = (int64_t)blockIdx.y * DB;
-    const int64_t nd = batch - d0 < DB ? batch - d0 : DB;
-    const bool staged = cells > 0 && cells * nd * (int64_t)sizeof(T) <= lds_bytes;
+    const int32_t nd = (int32_t)(batch - d0 < DB ? batch - d0 : DB);
+    const bool staged = cells > 0 && cells * DB * (int64_t)sizeof(T) <= lds_bytes;
     if (staged) {
-        stage_box<T, BY_COLUMN>(patch, x + d0 * ncols + (int64_t)i0 * src_gy + j0, ncols, src_gy,
-                                rows, cols, (int32_t)nd);
+        stage_cells<T, DB>(patch, x + d0 * ncols + (int64_t)i0 * src_gy + j0, ncols, src_gy,
+                           rows, cols, nd);
         __syncthreads();
     }
     const int32_t ti = (int32_t)(tile / G.ntx), tj = (int32_t)(tile - (int64_t)ti * G.ntx);
@@ -300,7 +265,6 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_apply_planned(
 #pragma unroll
     for (int m = 0; m < DB; m++) acc[m] = T(0);
     if (staged) {
-        const int32_t ncell = (int32_t)cells;
         int32_t q = s;
         // four entries' loads in flight at once; the terms are still added
         // one at a time in column order
@@ -313,17 +277,17 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_apply_planned(
                 wq[u] = w[q + u];
             }
 #pragma unroll
-            for (int u = 0; u < 4; u++)
+            for (int u = 0; u < 4; u++) {
+                const CellVals<T, DB> cv = patch[o[u]];
 #pragma unroll
-                for (int m = 0; m < DB; m++)
-                    if (m < nd) acc[m] = acc[m] + wq[u] * patch[m * ncell + o[u]];
+                for (int m = 0; m < DB; m++) acc[m] = acc[m] + wq[u] * cv.v[m];
+            }
         }
         for (; q < e; q++) {
-            const int32_t o = off[q];
+            const CellVals<T, DB> cv = patch[off[q]];
             const T wq = w[q];
 #pragma unroll
-            for (int m = 0; m < DB; m++)
-                if (m < nd) acc[m] = acc[m] + wq * patch[m * ncell + o];
+            for (int m = 0; m < DB; m++) acc[m] = acc[m] + wq * cv.v[m];
         }
     } else {
         const T* xb = x + d0 * ncols;
@@ -346,7 +310,7 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_apply_planned(
 // images the block stages the input patch in LDS and gathers from it.  The
 // entries and row pointers are read once per launch instead of once per
 // image group, and the blocks (one per tile) stay resident longer.
-template <class T, int DB, int MAXE, bool BY_COLUMN>
+template <class T, int DB, int MAXE>
 __global__ __launch_bounds__(APPLY_BLOCK) void k_apply_rounds(
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
     const uint16_t* __restrict__ off, const int32_t* __restrict__ box,
@@ -355,7 +319,7 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_apply_rounds(
     int64_t lds_bytes)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    T* patch = reinterpret_cast<T*>(lds_raw);
+    CellVals<T, DB>* patch = reinterpret_cast<CellVals<T, DB>*>(lds_raw);
     const int64_t nb = gridDim.x;  // a multiple of NUM_XCD
     const int64_t b = blockIdx.x;
     const int64_t tile = (b % NUM_XCD) * (nb / NUM_XCD) + b / NUM_XCD;
@@ -386,10 +350,10 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_apply_rounds(
     }
     const bool fits = ncell > 0 && (int64_t)ncell * DB * (int64_t)sizeof(T) <= lds_bytes;
     for (int64_t d0 = 0; d0 < batch; d0 += DB) {
-        const int64_t nd = batch - d0 < DB ? batch - d0 : DB;
+        const int32_t nd = (int32_t)(batch - d0 < DB ? batch - d0 : DB);
         if (fits) {
-            stage_box<T, BY_COLUMN>(patch, x + d0 * ncols + (int64_t)i0 * src_gy + j0, ncols,
-                                    src_gy, rows, cols, (int32_t)nd);
+            stage_cells<T, DB>(patch, x + d0 * ncols + (int64_t)i0 * src_gy + j0, ncols, src_gy,
+                               rows, cols, nd);
             __syncthreads();
         }
         if (valid) {
@@ -397,20 +361,20 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_apply_rounds(
 #pragma unroll
             for (int m = 0; m < DB; m++) acc[m] = T(0);
             if (fits) {
+                // (the accumulators of images past nd add 0 and are not stored)
 #pragma unroll
                 for (int q = 0; q < MAXE; q++) {
                     if (q < cnt) {
+                        const CellVals<T, DB> cv = patch[ro[q]];
 #pragma unroll
-                        for (int m = 0; m < DB; m++)
-                            if (m < nd) acc[m] = acc[m] + rw[q] * patch[m * ncell + ro[q]];
+                        for (int m = 0; m < DB; m++) acc[m] = acc[m] + rw[q] * cv.v[m];
                     }
                 }
                 for (int32_t q = s + MAXE; q < e; q++) {
-                    const int32_t o = off[q];
+                    const CellVals<T, DB> cv = patch[off[q]];
                     const T wq = w[q];
 #pragma unroll
-                    for (int m = 0; m < DB; m++)
-                        if (m < nd) acc[m] = acc[m] + wq * patch[m * ncell + o];
+                    for (int m = 0; m < DB; m++) acc[m] = acc[m] + wq * cv.v[m];
                 }
             } else {
                 const T* xg = x + d0 * ncols;
@@ -503,14 +467,22 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_apply_jobs(
 
 namespace {
 
-int g_tile_w = 16;          // tuned on the plain 2049^2 -> 1537^2 pair
+// Tile width for 8-byte values; 4-byte values use tiles twice as wide (up to
+// APPLY_BLOCK), so a staged patch row holds about the same bytes either way.
+// Tuned on the plain 2049^2 -> 1537^2 pair, 16 images: FP64 16 wide, FP32 32
+// wide (0.149 -> 0.129 ms forward, 0.181 -> 0.148 ms adjoint against 16).
+int g_tile_w = 16;
 int g_images_per_pass = 8;
-int64_t g_lds_limit = LDS_BYTES;
+// staged patch per block above which fewer images are staged per pass: at
+// 32 KiB the FP64 forward apply of the plain 2049^2 -> 1537^2 pair stages 4
+// images per pass and holds four blocks per CU instead of three (5 % faster);
+// FP32 and the adjoints stage 8 either way
+int64_t g_lds_limit = 32 * 1024;
 bool g_use_plan = true;
 bool g_rounds = true;  // one block per tile looping over image groups
 std::mutex g_plan_mu;
 
-ApplyGrid make_grid(const int64_t* shape, int64_t nrows)
+ApplyGrid make_grid(const int64_t* shape, int64_t nrows, int64_t value_bytes)
 {
     ApplyGrid G;
     G.gx = shape[0];
@@ -519,7 +491,7 @@ ApplyGrid make_grid(const int64_t* shape, int64_t nrows)
         G.gx = 1;
         G.gy = nrows;
     }
-    G.tw = g_tile_w;
+    G.tw = value_bytes >= 8 ? g_tile_w : std::min<int64_t>(2 * g_tile_w, APPLY_BLOCK);
     if (G.gx == 1 || G.tw > G.gy) G.tw = APPLY_BLOCK;
     G.th = APPLY_BLOCK / G.tw;
     G.ntx = (G.gy + G.tw - 1) / G.tw;
@@ -529,7 +501,7 @@ ApplyGrid make_grid(const int64_t* shape, int64_t nrows)
 
 // The plan of one direction for the current tile shape (made on first use
 // on stream s; later applies on other streams must be ordered after it).
-int ensure_plan(rg_weights* W, int dir, const int32_t* rowptr,
+int ensure_plan(rg_weights* W, int dir, int64_t value_bytes, const int32_t* rowptr,
                 const int32_t* col, const ApplyGrid& G, int64_t src_gy,
                 hipStream_t s, rg_weights::Plan** out)
 {
@@ -539,7 +511,7 @@ int ensure_plan(rg_weights* W, int dir, const int32_t* rowptr,
     if (!g_use_plan || G.gx == 1 || W->nnz == 0 || G.gx * G.gy < PLAN_MIN_ROWS)
         return RG_OK;
     std::lock_guard<std::mutex> lk(g_plan_mu);
-    rg_weights::Plan& P = W->plan[dir];
+    rg_weights::Plan& P = W->plan[2 * dir + (value_bytes < 8 ? 1 : 0)];
     if (P.tile_w != G.tw || P.ntiles != G.ntiles) {
         dev_free(P.box);
         dev_free(P.off);
@@ -569,9 +541,9 @@ int launch_apply(rg_weights* W, int dir, const int32_t* rowptr,
                  int64_t src_gy, int64_t batch, hipStream_t s)
 {
     if (nrows == 0 || batch == 0) return RG_OK;
-    const ApplyGrid G = make_grid(shape, nrows);
+    const ApplyGrid G = make_grid(shape, nrows, sizeof(T));
     rg_weights::Plan* P = nullptr;
-    RG_TRY(ensure_plan(W, dir, rowptr, col, G, src_gy, s, &P));
+    RG_TRY(ensure_plan(W, dir, sizeof(T), rowptr, col, G, src_gy, s, &P));
     const int64_t nb = (G.ntiles + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
     int db = g_images_per_pass;
     while (db > 1 && db > batch) db >>= 1;
@@ -585,14 +557,8 @@ int launch_apply(rg_weights* W, int dir, const int32_t* rowptr,
         if (lds > LDS_BYTES) lds = LDS_BYTES;
         dim3 g1((unsigned)nb);
 #define RG_ROUNDS(D)                                                                   \
-    do {                                                                               \
-        if (dir == 0)                                                                  \
-            k_apply_rounds<T, D, 8, true><<<g1, bl, (size_t)lds, s>>>(                 \
-                rowptr, col, P->off, P->box, w, x, y, nrows, ncols, src_gy, batch, G, lds); \
-        else                                                                           \
-            k_apply_rounds<T, D, 8, false><<<g1, bl, (size_t)lds, s>>>(                \
-                rowptr, col, P->off, P->box, w, x, y, nrows, ncols, src_gy, batch, G, lds); \
-    } while (0)
+    k_apply_rounds<T, D, 8><<<g1, bl, (size_t)lds, s>>>(                               \
+        rowptr, col, P->off, P->box, w, x, y, nrows, ncols, src_gy, batch, G, lds)
         switch (db) {
         case 16: RG_ROUNDS(16); break;
         case 8: RG_ROUNDS(8); break;
@@ -606,14 +572,8 @@ int launch_apply(rg_weights* W, int dir, const int32_t* rowptr,
         lds = (lds + 255) / 256 * 256;
         if (lds > LDS_BYTES) lds = LDS_BYTES;
 #define RG_PLANNED(D)                                                                  \
-    do {                                                                               \
-        if (dir == 0)                                                                  \
-            k_apply_planned<T, D, true><<<g, bl, (size_t)lds, s>>>(                    \
-                rowptr, col, P->off, P->box, w, x, y, nrows, ncols, src_gy, batch, G, lds); \
-        else                                                                           \
-            k_apply_planned<T, D, false><<<g, bl, (size_t)lds, s>>>(                   \
-                rowptr, col, P->off, P->box, w, x, y, nrows, ncols, src_gy, batch, G, lds); \
-    } while (0)
+    k_apply_planned<T, D><<<g, bl, (size_t)lds, s>>>(                                  \
+        rowptr, col, P->off, P->box, w, x, y, nrows, ncols, src_gy, batch, G, lds)
         switch (db) {
         case 16: RG_PLANNED(16); break;
         case 8: RG_PLANNED(8); break;

@@ -170,13 +170,15 @@ struct rg_weights {
     float* val_in_f32 = nullptr;    // lazily made single-precision copies
     float* val_out_f32 = nullptr;
     // apply plans (see rg_apply.hip), one per direction, made on first use
+    // One per direction and value size (index 2 * dir + (4-byte values)):
+    // FP32 applies use tiles twice as wide as FP64 ones.
     struct Plan {
         int64_t tile_w = 0;        // tile shape the plan was made for
         int32_t* box = nullptr;    // per tile: i0, j0, rows, cols of the patch
         uint16_t* off = nullptr;   // per CSR entry: offset inside its patch
         int64_t ntiles = 0;
         int64_t max_cells = 0;     // largest patch (cells)
-    } plan[2];
+    } plan[4];
     rg_build_stats stats = {};
     // set when the six CSR arrays are slices of a batched build's buffers
     std::shared_ptr<rg_arena> arena;

@@ -181,7 +181,7 @@ def test_apply_launch_shapes_same_bits(tile_w, ipp, lds_kib, rounds):
         assert np.array_equal(_apply(h, x), ref)
         assert np.array_equal(_apply(h, y, transpose=True), ref_t)
     finally:
-        check(lib().rg_set_apply_tuning(16, 8, 64, 1))
+        check(lib().rg_set_apply_tuning(16, 8, 32, 1))
 
 
 def test_apply_batched_matches_single():
