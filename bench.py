@@ -244,7 +244,8 @@ def main() -> int:
     scatter_bytes = 16 * T + 16 * T + 12 * nchunks + 8 * 4 * n_in
     reduce_bytes = 16 * T + 8 * (4 * n_in + 1) + 8 * n_in + 16 * nnz + 4 * n_in
     rooflines = [
-        {"kernel": "k_apply_rounds<double,8,8> (CSR gather apply, FP64, 16 images)", "bound": "hbm",
+        {"kernel": "k_apply_rounds (CSR gather apply from staged cell-major patches, FP64, 16 images)",
+         "bound": "hbm",
          "achieved": round(gbps64, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
          "frac": round(gbps64 / HBM_PEAK_GBPS, 4), "traffic": None,
          "bytes_per_launch": by64, "ms_per_launch": round(ms_apply64, 5)},
@@ -646,7 +647,7 @@ def _lat_stats(lat):
             "mean_ms": round(float(lat.mean()), 3)}
 
 
-SERVE_SWEEP_RATES = (1000, 1500, 2000, 2500, 3000, 3500)
+SERVE_SWEEP_RATES = (1000, 1500, 2000, 2500, 3000, 3500, 4000)
 
 
 def bench_serve(args) -> dict:
@@ -658,8 +659,11 @@ def bench_serve(args) -> dict:
     * `rate_sweep`: Poisson arrivals at rising rates (SERVE_SWEEP_RATES), 4 s
       each: the highest offered rate whose p99 stays within P99_BOUND_MS and
       whose throughput keeps up with the arrivals;
-    * `capacity`: then a closed loop keeping 256 requests in flight for 8 s:
-      the loop's maximum sustained request rate;
+    * `capacity`: then a closed loop keeping 256 requests in flight for 8 s;
+      the capacity reported is the highest sustained rate of the closed loop
+      and of the sweep's rates whose throughput kept up (the closed loop
+      releases results in submission order, so one slow request holds the
+      window and it can sustain less than an open loop does);
     * `plain`: the same request mix served one request per call (build a
       fresh pair with rg_build_conservative_2d, apply with rg_apply_*,
       synchronous copies; the 64 reusable pairs' weights kept): the
@@ -716,10 +720,13 @@ def bench_serve(args) -> dict:
         log(f"serving sweep at {rate}/s: p50 {r['p50_ms']} ms, p99 {r['p99_ms']} ms, "
             f"{r['throughput_req_per_s']} req/s; free device memory {free_gb()} GiB")
     cap = leg(n_max=10**9, seconds=8.0, window=256)
-    capacity = cap["throughput_req_per_s"]
-    log(f"serving capacity (256 in flight): {cap}; free device memory {free_gb()} GiB")
-    ok = [r for r in sweep if r["p99_ms"] <= P99_BOUND_MS
-          and r["throughput_req_per_s"] >= 0.97 * r["offered_req_per_s"]]
+    log(f"serving closed loop (256 in flight): {cap}; free device memory {free_gb()} GiB")
+    kept = [r for r in sweep if r["throughput_req_per_s"] >= 0.97 * r["offered_req_per_s"]]
+    ok = [r for r in kept if r["p99_ms"] <= P99_BOUND_MS]
+    best = max([cap] + kept, key=lambda r: r["throughput_req_per_s"])
+    capacity = best["throughput_req_per_s"]
+    capacity_leg = ("closed loop, 256 in flight" if best is cap
+                    else f"open loop at {best['offered_req_per_s']:.0f}/s")
     st = srv.stats()
     srv.close()
     pcie = pcie_bandwidth(ring)
@@ -729,8 +736,9 @@ def bench_serve(args) -> dict:
                     "arrival; per-axis vertices log-uniform in [33,1025]; 1-32 value arrays, "
                     "FP32/FP64 50/50",
         "offered": offered,
-        "capacity": cap,
+        "closed_loop": cap,
         "capacity_req_per_s": capacity,
+        "capacity_leg": capacity_leg,
         "rate_sweep": sweep,
         "p99_bound_ms": P99_BOUND_MS,
         "max_rate_within_p99_bound": (max(r["offered_req_per_s"] for r in ok) if ok else None),
@@ -739,7 +747,7 @@ def bench_serve(args) -> dict:
         # the loop moves every request's values in and results out over
         # PCIe: its roofline is the copy engines' bandwidth
         "pcie": pcie,
-        "capacity_pcie_frac": round((cap["h2d_GBps"] + cap["d2h_GBps"]) / pcie["both_GBps"], 3),
+        "capacity_pcie_frac": round((best["h2d_GBps"] + best["d2h_GBps"]) / pcie["both_GBps"], 3),
         "max_group": st["max_group_seen"],
         "peak_admitted_GB": round(st["peak_admitted_bytes"] / 1e9, 3),
         # the figures of the 2000/s run at the top level, as before
@@ -749,7 +757,7 @@ def bench_serve(args) -> dict:
         "throughput_req_per_s": offered["throughput_req_per_s"],
         "failed": st["failed"],
     }
-    log(f"serving: capacity {capacity} req/s, plain {plain['req_per_s']} req/s, "
+    log(f"serving: capacity {capacity} req/s ({capacity_leg}), plain {plain['req_per_s']} req/s, "
         f"x{out['served_vs_plain']}; max rate within p99 <= {P99_BOUND_MS} ms: "
         f"{out['max_rate_within_p99_bound']}")
     return out
