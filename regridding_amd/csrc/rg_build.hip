@@ -1168,7 +1168,12 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
 // source slot: two short dependent chains per run instead of one per
 // record.
 constexpr int HEAVY_CAP = RED_CAP;  // LDS entries per wavefront
-constexpr int HEAVY_UNROLL = 2;  // more raises the kernel past 256 registers
+constexpr int HEAVY_UNROLL = 4;  // records loaded per lane at once
+constexpr int HEAVY_SCAN = 8;    // cell bytes requested per lane at once
+// A group's cell bytes are kept in LDS from the counting scan when it has at
+// most this many records (most heavy groups), so the scans of its runs read
+// LDS instead of memory; 2 KiB per wavefront keeps four blocks per CU.
+constexpr int HEAVY_CELLB = 2048;
 
 __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_heavy(
     const int64_t* __restrict__ bucket_off, int64_t n_groups,
@@ -1185,6 +1190,7 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_heavy(
     __shared__ uint64_t sk[RED_WAVES][HEAVY_CAP];
     __shared__ double sv[RED_WAVES][HEAVY_CAP];
     __shared__ uint8_t sc[RED_WAVES][HEAVY_CAP];
+    __shared__ uint8_t scb[RED_WAVES][HEAVY_CELLB];
     __shared__ int32_t scnt[RED_WAVES][64];
     const int wid = block_wave(), lane = threadIdx.x & 63;
     int32_t* C = scnt[wid];
@@ -1192,15 +1198,18 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_heavy(
     double* V = sv[wid];
     int64_t* SRC = reinterpret_cast<int64_t*>(sv[wid]);  // a run's record sources
     uint8_t* SC = sc[wid];
+    uint8_t* CB = scb[wid];  // the group's cell bytes, by record
     constexpr int U = HEAVY_CAP / 64;
     const int32_t nh = *n_heavy;
+    // the next list position, requested one group ahead
+    int32_t hn = 0;
+    if (lane == 0) hn = atomicAdd(heavy_next, 1);
     for (;;) {
-        int32_t h = 0;
-        if (lane == 0) h = atomicAdd(heavy_next, 1);
         // (all lanes are active here: the first active lane is lane 0, and
         // the group's tables are then read with scalar loads)
-        h = __builtin_amdgcn_readfirstlane(h);
+        const int32_t h = __builtin_amdgcn_readfirstlane(hn);
         if (h >= nh) break;  // uniform per wavefront
+        if (lane == 0) hn = atomicAdd(heavy_next, 1);
         const int64_t grp = heavy_list[h];
         const int64_t i = grp * 64 + lane;
         const double vol = vol_in[i];
@@ -1209,16 +1218,20 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_heavy(
         const int64_t total = G.total;
         int32_t* hist_out =
             hist_out_all + pcs[__builtin_amdgcn_readfirstlane(pair_of_group[grp])].out_gbase;
+        const bool cached = total <= HEAVY_CELLB;
         C[lane] = 0;
         wave_sync_lds();
-        for (int64_t q0 = lane; q0 < total; q0 += 64 * HEAVY_UNROLL) {
-            int32_t cb[HEAVY_UNROLL];
+        for (int64_t q0 = lane; q0 < total; q0 += 64 * HEAVY_SCAN) {
+            int32_t cb[HEAVY_SCAN];
 #pragma unroll
-            for (int u = 0; u < HEAVY_UNROLL; u++)
+            for (int u = 0; u < HEAVY_SCAN; u++)
                 cb[u] = q0 + 64 * u < total ? rcell[G.src(q0 + 64 * u)] : -1;
 #pragma unroll
-            for (int u = 0; u < HEAVY_UNROLL; u++)
-                if (cb[u] >= 0) atomicAdd(&C[cb[u]], 1);
+            for (int u = 0; u < HEAVY_SCAN; u++)
+                if (cb[u] >= 0) {
+                    atomicAdd(&C[cb[u]], 1);
+                    if (cached) CB[q0 + 64 * u] = (uint8_t)cb[u];
+                }
         }
         wave_sync_lds();
         const int32_t n = C[lane];
@@ -1237,7 +1250,8 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_heavy(
                 wave_sync_lds();
                 for (int64_t q = lane; q < total; q += 64) {
                     const int64_t src = G.src(q);
-                    if (rcell[src] == c) red[base + sb + atomicAdd(&C[0], 1)] = rec[src];
+                    if ((cached ? CB[q] : rcell[src]) == c)
+                        red[base + sb + atomicAdd(&C[0], 1)] = rec[src];
                 }
                 if (lane == c) long_list[atomicAdd(n_long, 1)] = (int32_t)i | INT32_MIN;
                 wave_sync_lds();
@@ -1249,13 +1263,15 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_heavy(
             C[lane] = s - sb;  // placement cursors of the run's cells
             wave_sync_lds();
             // the run's records, cell-major in LDS: their sources first ...
-            for (int64_t q0 = lane; q0 < total; q0 += 64 * HEAVY_UNROLL) {
-                int32_t cb[HEAVY_UNROLL];
+            for (int64_t q0 = lane; q0 < total; q0 += 64 * HEAVY_SCAN) {
+                int32_t cb[HEAVY_SCAN];
 #pragma unroll
-                for (int u = 0; u < HEAVY_UNROLL; u++)
-                    cb[u] = q0 + 64 * u < total ? rcell[G.src(q0 + 64 * u)] : -1;
+                for (int u = 0; u < HEAVY_SCAN; u++) {
+                    const int64_t q = q0 + 64 * u;
+                    cb[u] = q < total ? (cached ? CB[q] : rcell[G.src(q)]) : -1;
+                }
 #pragma unroll
-                for (int u = 0; u < HEAVY_UNROLL; u++)
+                for (int u = 0; u < HEAVY_SCAN; u++)
                     if (cb[u] >= c && cb[u] < d) {
                         const int32_t p = atomicAdd(&C[cb[u]], 1);
                         SRC[p] = G.src(q0 + 64 * u);
