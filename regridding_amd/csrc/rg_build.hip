@@ -1111,8 +1111,14 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
         pos[u] = -1;
         if (cc[u] >= 0) {
             const uint64_t k = rr[u].key;
-            int32_t r = 0;
-            for (int32_t j = cs; j < cs + cn; j++) r += K[j] < k ? 1 : 0;
+            // four keys requested before any is compared
+            int32_t r = 0, j = cs;
+            const int32_t je = cs + cn;
+            for (; j + 4 <= je; j += 4) {
+                const uint64_t k0 = K[j], k1 = K[j + 1], k2 = K[j + 2], k3 = K[j + 3];
+                r += (k0 < k ? 1 : 0) + (k1 < k ? 1 : 0) + (k2 < k ? 1 : 0) + (k3 < k ? 1 : 0);
+            }
+            for (; j < je; j++) r += K[j] < k ? 1 : 0;
             pos[u] = cs + r;
         }
     }
@@ -1311,8 +1317,15 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_heavy(
                 pe[u] = -1;
                 if (e < cnt) {
                     const uint64_t k = K[e];
-                    int32_t r = 0;
-                    for (int32_t j = cs; j < cs + cn; j++) r += K[j] < k ? 1 : 0;
+                    // four keys requested before any is compared (one LDS
+                    // latency per four keys, not per key)
+                    int32_t r = 0, j = cs;
+                    const int32_t je = cs + cn;
+                    for (; j + 4 <= je; j += 4) {
+                        const uint64_t k0 = K[j], k1 = K[j + 1], k2 = K[j + 2], k3 = K[j + 3];
+                        r += (k0 < k ? 1 : 0) + (k1 < k ? 1 : 0) + (k2 < k ? 1 : 0) + (k3 < k ? 1 : 0);
+                    }
+                    for (; j < je; j++) r += K[j] < k ? 1 : 0;
                     ke[u] = k;
                     ve[u] = weight_of(V[e], cv, wi, grp * 64 + ce);
                     pe[u] = cs + r;

@@ -760,30 +760,76 @@ RG_HD void sort_records(KV* r, int64_t n)
 RG_HD int64_t bucket_reduce_sorted(uint64_t* keys, double* vals, int64_t n,
                                    bool* long_run, int64_t st = 1)
 {
-    int64_t m = 0, p = 0;
-    bool lr = false;
-    while (p < n) {
-        uint64_t out = keys[p * st] >> 32;
-        int64_t q = p + 1;
-        while (q < n && (keys[q * st] >> 32) == out) q++;
-        if (q - p - 1 > 128) lr = true;
-        m++;
-        p = q;
+    // Both passes stream over the terms and load them four at a time: a scan
+    // that loads the next key only after comparing the previous one waits out
+    // one memory (LDS) latency per term.  First the run lengths: a run of more
+    // than 129 terms leaves the cell sorted and unreduced for the long-run
+    // pass, and then the number of runs is returned.
+    int64_t run = 0, longest = 0, m = 0;
+    uint64_t prev = 0;
+    for (int64_t t0 = 0; t0 < n; t0 += 4) {
+        uint64_t o[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) o[u] = t0 + u < n ? keys[(t0 + u) * st] >> 32 : 0;
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (t0 + u < n) {
+                const bool same = t0 + u > 0 && o[u] == prev;
+                run = same ? run + 1 : 0;
+                m += same ? 0 : 1;
+                longest = run > longest ? run : longest;
+                prev = o[u];
+            }
     }
-    *long_run = lr;
-    if (lr) return m;
+    *long_run = longest > 128;
+    if (*long_run) return m;
+    // Then each run becomes its first term plus the sum of the others.  Under
+    // 8 others that sum is numpy's sequential one seeded with -0.0 and is
+    // accumulated as the terms stream by; a longer run sums its terms with
+    // pairwise_leaf, still in place (a run's output goes to a slot at or
+    // before its first term, after its terms are read).
     m = 0;
-    p = 0;
-    while (p < n) {
-        uint64_t out = keys[p * st] >> 32;
-        int64_t q = p + 1;
-        while (q < n && (keys[q * st] >> 32) == out) q++;
-        double v = q - p == 1 ? vals[p * st]
-                              : vals[p * st] + pairwise_leaf(vals + (p + 1) * st, q - p - 1, st);
-        keys[m * st] = out;
-        vals[m * st] = v;
+    int64_t p = 0, cnt = 0;
+    uint64_t cur = 0;
+    double first = 0, res = -0.0;
+    for (int64_t t0 = 0; t0 < n; t0 += 4) {
+        uint64_t o[4];
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (t0 + u < n) {
+                o[u] = keys[(t0 + u) * st] >> 32;
+                v[u] = vals[(t0 + u) * st];
+            }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int64_t t = t0 + u;
+            if (t >= n) continue;
+            if (t > 0 && o[u] == cur) {
+                cnt++;
+                res += v[u];
+                continue;
+            }
+            if (t > 0) {
+                keys[m * st] = cur;
+                vals[m * st] = cnt == 0  ? first
+                               : cnt < 8 ? first + res
+                                         : first + pairwise_leaf(vals + (p + 1) * st, cnt, st);
+                m++;
+            }
+            cur = o[u];
+            first = v[u];
+            res = -0.0;
+            cnt = 0;
+            p = t;
+        }
+    }
+    if (n > 0) {
+        keys[m * st] = cur;
+        vals[m * st] = cnt == 0  ? first
+                       : cnt < 8 ? first + res
+                                 : first + pairwise_leaf(vals + (p + 1) * st, cnt, st);
         m++;
-        p = q;
     }
     return m;
 }

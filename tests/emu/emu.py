@@ -100,6 +100,22 @@ def reduceat_segment(a) -> float:
     return float(lib().rge_reduceat_segment(_d(a), a.size))
 
 
+def bucket_reduce(keys, vals, stride: int = 1):
+    """bucket_reduce_sorted on a strided copy of one sorted bucket: returns
+    (count, long_run, keys, vals) with the bucket as the function left it."""
+    n = len(keys)
+    kb = np.full(max(n * stride, 1), np.uint64(0xDEADBEEFDEADBEEF), dtype=np.uint64)
+    vb = np.full(max(n * stride, 1), np.nan)
+    kb[: n * stride: stride] = keys
+    vb[: n * stride: stride] = vals
+    lr = ctypes.c_int32(0)
+    L = lib()
+    L.rge_bucket_reduce.restype = _i64
+    L.rge_bucket_reduce.argtypes = [ctypes.c_void_p, _pd, _i64, _i64, ctypes.POINTER(ctypes.c_int32)]
+    m = L.rge_bucket_reduce(kb.ctypes.data, _d(vb), n, stride, ctypes.byref(lr))
+    return int(m), bool(lr.value), kb[: n * stride: stride].copy(), vb[: n * stride: stride].copy()
+
+
 def reduceat_segment_inplace(a, stride: int = 1) -> float:
     """The stackless in-place variant (on a strided copy of ``a``)."""
     a = np.asarray(a, dtype=np.float64)

@@ -435,6 +435,17 @@ double rge_reduceat_segment_inplace(double* a, int64_t n, int64_t stride)
     return n == 1 ? a[0] : a[0] + pairwise_sum_inplace(a + stride, n - 1, stride);
 }
 
+// bucket_reduce_sorted on one sorted bucket (entry j at keys[j * stride]):
+// returns the count of unique outputs, or of runs when *long_run is set.
+int64_t rge_bucket_reduce(uint64_t* keys, double* vals, int64_t n, int64_t stride,
+                          int32_t* long_run)
+{
+    bool lr = false;
+    const int64_t m = bucket_reduce_sorted(keys, vals, n, &lr, stride);
+    *long_run = lr ? 1 : 0;
+    return m;
+}
+
 void rge_cell_area(const double* x, const double* y, int64_t nx, int64_t ny,
                    double* out)
 {
