@@ -177,14 +177,38 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_plan(
         s = rowptr[r];
         e = rowptr[r + 1];
     }
+    // The row's first PLAN_E entries are loaded once, all at once, and kept
+    // for both passes (a row holds about five at the usual grid ratios);
+    // columns and the row length fit 32 bits, so the divisions are 32-bit.
+    constexpr int PLAN_E = 8;
+    const uint32_t gy = (uint32_t)src_gy;
+    uint32_t cr[PLAN_E];
+#pragma unroll
+    for (int u = 0; u < PLAN_E; u++) cr[u] = s + u < e ? (uint32_t)col[s + u] : 0u;
     int32_t a0 = INT32_MAX, a1 = INT32_MIN, b0 = INT32_MAX, b1 = INT32_MIN;
-    for (int32_t q = s; q < e; q++) {
-        const int32_t c = col[q];
-        const int32_t ci = (int32_t)(c / src_gy), cj = (int32_t)(c - ci * src_gy);
+#pragma unroll
+    for (int u = 0; u < PLAN_E; u++)
+        if (s + u < e) {
+            const int32_t ci = (int32_t)(cr[u] / gy), cj = (int32_t)(cr[u] - (uint32_t)ci * gy);
+            a0 = min(a0, ci); a1 = max(a1, ci);
+            b0 = min(b0, cj); b1 = max(b1, cj);
+        }
+    for (int32_t q = s + PLAN_E; q < e; q++) {
+        const uint32_t c = (uint32_t)col[q];
+        const int32_t ci = (int32_t)(c / gy), cj = (int32_t)(c - (uint32_t)ci * gy);
         a0 = min(a0, ci); a1 = max(a1, ci);
         b0 = min(b0, cj); b1 = max(b1, cj);
     }
-    if (s < e) {
+    // the wavefront's extremes first (shuffles), then one LDS atomic per
+    // wavefront and bound: 256 threads' atomics on the same four words
+    // serialize in the LDS and made this kernel (a row without entries holds
+    // the neutral values)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        a0 = min(a0, __shfl_xor(a0, o, 64)); a1 = max(a1, __shfl_xor(a1, o, 64));
+        b0 = min(b0, __shfl_xor(b0, o, 64)); b1 = max(b1, __shfl_xor(b1, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0 && a0 <= a1) {
         atomicMin(&lo_i, a0); atomicMax(&hi_i, a1);
         atomicMin(&lo_j, b0); atomicMax(&hi_j, b1);
     }
@@ -200,26 +224,44 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_plan(
     }
     // offsets are only read for patches that fit the LDS (< 2^16 cells)
     const int64_t cells = (int64_t)rows * cols;
-    for (int32_t q = s; q < e; q++) {
-        const int32_t c = col[q];
-        const int32_t ci = (int32_t)(c / src_gy), cj = (int32_t)(c - ci * src_gy);
+#pragma unroll
+    for (int u = 0; u < PLAN_E; u++)
+        if (s + u < e) {
+            const int32_t ci = (int32_t)(cr[u] / gy), cj = (int32_t)(cr[u] - (uint32_t)ci * gy);
+            const int64_t o = (int64_t)(ci - lo_i) * cols + (cj - lo_j);
+            off[s + u] = cells < 65536 ? (uint16_t)o : (uint16_t)0;
+        }
+    for (int32_t q = s + PLAN_E; q < e; q++) {
+        const uint32_t c = (uint32_t)col[q];
+        const int32_t ci = (int32_t)(c / gy), cj = (int32_t)(c - (uint32_t)ci * gy);
         const int64_t o = (int64_t)(ci - lo_i) * cols + (cj - lo_j);
         off[q] = cells < 65536 ? (uint16_t)o : (uint16_t)0;
     }
 }
 
-__global__ void k_plan_max(const int32_t* __restrict__ box, int64_t ntiles,
-                           int64_t* __restrict__ out)
+// One block of PLAN_MAX_BLOCK threads: the largest patch over all tiles
+// (each thread's share of the tiles is loaded four at a time).
+constexpr int PLAN_MAX_BLOCK = 1024;
+
+__global__ __launch_bounds__(PLAN_MAX_BLOCK) void k_plan_max(const int32_t* __restrict__ box,
+                                                             int64_t ntiles,
+                                                             int64_t* __restrict__ out)
 {
     int64_t m = 0;
-    for (int64_t t = threadIdx.x; t < ntiles; t += blockDim.x) {
-        const int64_t c = (int64_t)box[4 * t + 2] * box[4 * t + 3];
-        m = c > m ? c : m;
+    for (int64_t t0 = threadIdx.x; t0 < ntiles; t0 += 4 * PLAN_MAX_BLOCK) {
+        int64_t c[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int64_t t = t0 + u * PLAN_MAX_BLOCK;
+            c[u] = t < ntiles ? (int64_t)box[4 * t + 2] * box[4 * t + 3] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) m = c[u] > m ? c[u] : m;
     }
-    __shared__ int64_t sh[256];
+    __shared__ int64_t sh[PLAN_MAX_BLOCK];
     sh[threadIdx.x] = m;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
+    for (int o = PLAN_MAX_BLOCK / 2; o > 0; o >>= 1) {
         if ((int)threadIdx.x < o && sh[threadIdx.x + o] > sh[threadIdx.x])
             sh[threadIdx.x] = sh[threadIdx.x + o];
         __syncthreads();
@@ -524,7 +566,7 @@ int ensure_plan(rg_weights* W, int dir, int64_t value_bytes, const int32_t* rowp
                                                         P.box, P.off);
         DevBuf<int64_t> mx;
         RG_TRY(mx.alloc(1, s));
-        k_plan_max<<<1, 256, 0, s>>>(P.box, G.ntiles, mx.p);
+        k_plan_max<<<1, PLAN_MAX_BLOCK, 0, s>>>(P.box, G.ntiles, mx.p);
         RG_HIP(hipGetLastError());
         RG_TRY(read_scalars(mx.p, &P.max_cells, sizeof(int64_t), s));
         P.tile_w = G.tw;
