@@ -9,7 +9,9 @@ every entry point that needs it raises ``RuntimeError``.
 from __future__ import annotations
 
 import ctypes
+import importlib.util
 import os
+import sys
 import threading
 import weakref
 
@@ -219,11 +221,22 @@ _lib = None
 def _load_torch_runtime_first() -> None:
     """PyTorch-ROCm carries its own HIP runtime under the soname this
     library links (libamdhip64.so.7), and a process can hold only one: the
-    first loaded serves both.  When torch is installed it is imported before
-    the library, so torch tensors and this library share torch's runtime (a
-    torch imported after the ROCm one finds no GPU)."""
-    if os.environ.get("RG_NO_TORCH"):
+    first loaded serves both (a torch imported after the ROCm one finds no
+    GPU).  When torch is installed, its runtime library is loaded first,
+    globally, from torch's own lib folder, without importing torch, so this
+    library and a torch imported later share it; if that load fails, torch
+    itself is imported."""
+    if os.environ.get("RG_NO_TORCH") or "torch" in sys.modules:
         return
+    spec = importlib.util.find_spec("torch")
+    if spec is None or not spec.submodule_search_locations:
+        return
+    runtime = os.path.join(list(spec.submodule_search_locations)[0], "lib", "libamdhip64.so")
+    try:
+        ctypes.CDLL(runtime, mode=ctypes.RTLD_GLOBAL)
+        return
+    except OSError:
+        pass
     try:
         import torch  # noqa: F401
     except ImportError:
