@@ -1053,9 +1053,11 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
             rr[u] = rec[src];
         }
     }
+    // (the count's returning atomic also gives each record its place in its
+    // cell, used by both layouts below)
+    int32_t dd[RED_CAP / 64];
 #pragma unroll
-    for (int u = 0; u < RED_CAP / 64; u++)
-        if (cc[u] >= 0) atomicAdd(&C[cc[u]], 1);
+    for (int u = 0; u < RED_CAP / 64; u++) dd[u] = cc[u] >= 0 ? atomicAdd(&C[cc[u]], 1) : 0;
     wave_sync_lds();
     // records per cell, and the cell-major offsets
     const int32_t n = C[lane];
@@ -1070,15 +1072,11 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
         // every cell fits RED_NET entries: entry j of cell c sits at
         // j * 64 + c, so the lanes (one per cell) touch consecutive LDS
         // words at every step of the sort and the reduction
-        wave_sync_lds();
-        C[lane] = 0;
-        wave_sync_lds();
 #pragma unroll
         for (int u = 0; u < RED_CAP / 64; u++)
             if (cc[u] >= 0) {
-                const int32_t d = atomicAdd(&C[cc[u]], 1);
-                K[d * 64 + cc[u]] = rr[u].key;
-                V[d * 64 + cc[u]] = rr[u].val;
+                K[dd[u] * 64 + cc[u]] = rr[u].key;
+                V[dd[u] * 64 + cc[u]] = rr[u].val;
             }
         wave_sync_lds();
         if (nmax <= RED_NET / 2)
@@ -1094,14 +1092,14 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_groups(
         return;
     }
     wave_sync_lds();
-    C[lane] = s;  // placement cursors
-    wave_sync_lds();
     // the keys cell-major in LDS; then every record's place in its cell is
     // the number of smaller keys of its cell (the keys are unique: they hold
     // the emission ordinal), and the records are written there, weighted
 #pragma unroll
-    for (int u = 0; u < RED_CAP / 64; u++)
-        if (cc[u] >= 0) K[atomicAdd(&C[cc[u]], 1)] = rr[u].key;
+    for (int u = 0; u < RED_CAP / 64; u++) {
+        const int32_t cs = __shfl(s, cc[u] >= 0 ? cc[u] : lane, 64);
+        if (cc[u] >= 0) K[cs + dd[u]] = rr[u].key;
+    }
     wave_sync_lds();
     int32_t pos[RED_CAP / 64];
 #pragma unroll
