@@ -25,6 +25,12 @@
 // neighbouring tiles in that XCD's L2.  The launch's second dimension is
 // the image group: all tiles of images [0, DB) first, then the next DB, so
 // the gathered footprint per pass is DB images deep.
+//
+// The results are stored nontemporal: each is written once and not read
+// back by the launch, so written through the caches they would only evict
+// the input patches that neighbouring tiles share (at 2049^2 -> 1537^2 and
+// 16 images: FP64 0.210 -> 0.205 ms, FP32 0.130 -> 0.123 ms, FP64 adjoint
+// 0.256 -> 0.248 ms, FP32 adjoint 0.148 -> 0.094 ms).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -77,7 +83,7 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_apply(
     T* yb = y + d0 * nrows + r;
 #pragma unroll
     for (int m = 0; m < DB; m++)
-        if (m < nd) yb[m * nrows] = acc[m];
+        if (m < nd) __builtin_nontemporal_store(acc[m], &yb[m * nrows]);
 }
 
 // ---- the apply plan -------------------------------------------------------
@@ -344,7 +350,7 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_apply_planned(
     T* yb = y + d0 * nrows + r;
 #pragma unroll
     for (int m = 0; m < DB; m++)
-        if (m < nd) yb[m * nrows] = acc[m];
+        if (m < nd) __builtin_nontemporal_store(acc[m], &yb[m * nrows]);
 }
 
 // The planned apply with all images in one block: the lane's CSR entries
@@ -431,7 +437,7 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_apply_rounds(
             T* yb = y + d0 * nrows + r;
 #pragma unroll
             for (int m = 0; m < DB; m++)
-                if (m < nd) yb[m * nrows] = acc[m];
+                if (m < nd) __builtin_nontemporal_store(acc[m], &yb[m * nrows]);
         }
         if (fits) __syncthreads();  // the next group overwrites the patch
     }

@@ -76,7 +76,10 @@ struct HistCache {
 
 // Counting walk: counts the chunk's triples, adds them to the per-input-cell
 // histogram and keeps the first `cap` of them in the staging buffer.  Call
-// flush() after the walk.
+// flush() after the walk.  The staged triples are stored nontemporal: the
+// staging (0.8 GB at 2049^2 -> 1537^2) is read once, by the scatter, long
+// after, and written through the caches it evicts the grid lines the walks
+// of neighbouring chunks re-read (the walk 0.82 -> 0.79 ms there).
 struct StageSink {
     int32_t n;
     int64_t g, cap;
@@ -92,7 +95,9 @@ struct StageSink {
             t.cin = (int32_t)cin;
             t.cout = (int32_t)cout;
             t.a = a;
-            stage[stage_index(g, n, cap)] = t;
+            typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+            __builtin_nontemporal_store(__builtin_bit_cast(u32x4, t),
+                                        (u32x4*)&stage[stage_index(g, n, cap)]);
         }
         n++;
         hc.add(hist, (int32_t)(p * ps + (cin >> 6)), 1);
