@@ -2015,7 +2015,7 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
             d.nrows = pcs[k].n_out;
             d.ncols = pcs[k].n_in;
             d.batch = J.batch;
-            d.nrt = (d.nrows + apply_block_rows() - 1) / apply_block_rows();
+            job_tiles(d, pairs[k].nxo - 1, pairs[k].nyo - 1, 8);
             d.block_base = gblocks;
             gprefix.push_back(gblocks);
             gblocks += d.nrt * ((d.batch + JOB_DB - 1) / JOB_DB);

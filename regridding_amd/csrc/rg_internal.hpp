@@ -130,7 +130,9 @@ __device__ inline int seg_of(const int64_t* __restrict__ base, int n, int64_t x)
 
 namespace rg {
 // One job of a batched apply (rg_apply.hip): CSR rows of one weight set
-// applied to `batch` value arrays; cut into blocks of rows x images.
+// applied to `batch` value arrays; cut into blocks of (tile of rows) x
+// images.  The rows are the cells of a (gx, gy) grid and a tile is tw x th
+// of them (job_tiles sets the tiling).
 struct JobDesc {
     const int32_t* rowptr;
     const int32_t* col;
@@ -140,14 +142,19 @@ struct JobDesc {
     int64_t nrows, ncols, batch;
     int64_t block_base;  // first block of this job
     int64_t nrt;         // row tiles of the job
+    int64_t gy, ntx;     // grid width (rows per grid row); tiles per tile row
+    int32_t tw, th;      // tile width and height (tw * th = block size)
 };
 constexpr int JOB_DB = 8;  // images per block
+// Tiles the job's rows, the cells of a (gx, gy) grid (gx * gy == nrows; any
+// other shape is taken as one long row), into blocks; sets gy, ntx, tw, th
+// and nrt.
+void job_tiles(JobDesc& d, int64_t gx, int64_t gy, int64_t value_bytes);
 // Launches the FP64 batched apply over a device job table (block prefix
 // `prefix`, `nblocks` blocks in all); every block returns at once when
 // *abort_flag is nonzero (abort_flag may be null).
 int apply_jobs_device(const JobDesc* jobs, const int64_t* prefix, int njobs, int64_t nblocks,
                       const int32_t* abort_flag, hipStream_t s);
-int64_t apply_block_rows();  // output rows per block of the batched apply
 }  // namespace rg
 
 struct rg_arena {
