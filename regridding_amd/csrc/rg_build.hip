@@ -888,7 +888,12 @@ __global__ __launch_bounds__(WALK_MAX_BLOCK) void k_scatter(const Build* __restr
 constexpr int32_t CAP_TRIPLES = 1, CAP_OVERFLOW = 2, CAP_WEIGHTS = 4, CAP_FAILED = 8,
                   CAP_MISMATCH = 16;
 
-constexpr int RED_WAVES = 2;     // wavefronts per block
+// wavefronts per block: one, so that the blocks pack the LDS more finely (a
+// block's 16.6 KiB per wavefront: nine wavefronts per CU instead of eight
+// in blocks of two; 0.547 -> 0.511 ms at 2049^2 -> 1537^2, the 256-pair
+// batch's reduce phase 7.31 -> 7.11 ms)
+constexpr int RED_WAVES = 1;
+constexpr int HEAVY_WAVES = 2;  // k_reduce_heavy's
 constexpr int RED_CAP = 1024;    // LDS entries per wavefront (16 KiB)
 constexpr int RED_NET = 16;      // largest cell sorted in registers
 
@@ -1184,7 +1189,7 @@ constexpr int HEAVY_SCAN = 8;    // cell bytes requested per lane at once
 // LDS instead of memory; 2 KiB per wavefront keeps four blocks per CU.
 constexpr int HEAVY_CELLB = 2048;
 
-__global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_heavy(
+__global__ __launch_bounds__(HEAVY_WAVES * 64) void k_reduce_heavy(
     const int64_t* __restrict__ bucket_off, int64_t n_groups,
     const int64_t* __restrict__ group_off, const KV* __restrict__ rec,
     const uint8_t* __restrict__ rcell, KV* __restrict__ red,
@@ -1196,11 +1201,11 @@ __global__ __launch_bounds__(RED_WAVES * 64) void k_reduce_heavy(
     const PairCells* __restrict__ pcs, const int32_t* __restrict__ pair_of_group,
     int32_t* __restrict__ hist_out_all)
 {
-    __shared__ uint64_t sk[RED_WAVES][HEAVY_CAP];
-    __shared__ double sv[RED_WAVES][HEAVY_CAP];
-    __shared__ uint8_t sc[RED_WAVES][HEAVY_CAP];
-    __shared__ uint8_t scb[RED_WAVES][HEAVY_CELLB];
-    __shared__ int32_t scnt[RED_WAVES][64];
+    __shared__ uint64_t sk[HEAVY_WAVES][HEAVY_CAP];
+    __shared__ double sv[HEAVY_WAVES][HEAVY_CAP];
+    __shared__ uint8_t sc[HEAVY_WAVES][HEAVY_CAP];
+    __shared__ uint8_t scb[HEAVY_WAVES][HEAVY_CELLB];
+    __shared__ int32_t scnt[HEAVY_WAVES][64];
     const int wid = block_wave(), lane = threadIdx.x & 63;
     int32_t* C = scnt[wid];
     uint64_t* K = sk[wid];
@@ -2250,7 +2255,7 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
                                               d_pcs, pair_of_ing.p, hist_out.p, T,
                                               counters.p + 8);
     // (resident waves: four blocks of two waves per CU fit the LDS)
-    k_reduce_heavy<<<1024, RED_WAVES * 64, 0, s>>>(
+    k_reduce_heavy<<<1024, HEAVY_WAVES * 64, 0, s>>>(
         bucket_off.p, n_groups, group_off.p, rec.p, rcell.p, red.p, cell_off.p, vol.p,
         wi_all.p, nnz_row.p, group_mode.p, long_list.p, counters.p + 3, heavy_list.p,
         counters.p + 5, counters.p + 9, d_pcs, pair_of_ing.p, hist_out.p);
