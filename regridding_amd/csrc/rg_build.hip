@@ -2275,12 +2275,18 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
     std::vector<int64_t> hbases(2 * (npairs + 1) + 4);
     int32_t cnt2[8] = {};
     int64_t nnz;
+    DevBuf<int64_t> off_out;
     if (capmode) {
         k_caps_check_weights<<<1, 1, 0, s>>>(row_off.p + cell_tot, counters.p, caps->nnz_cap);
         RG_HIP(hipGetLastError());
         nnz = caps->nnz_cap;
     } else {
-        RG_TRY(read_scalars(bases.p, hbases.data(), sizeof(int64_t) * hbases.size(), s));
+        // the output rows' scan does not depend on the weight count: it runs
+        // while the host waits for the count
+        RG_TRY(read_scalars_begin(bases.p, sizeof(int64_t) * hbases.size(), s));
+        RG_TRY(off_out.alloc(out_tot + 1, s));
+        RG_TRY(scan_i32_to_i64(hist_out.p, off_out.p, out_tot, s));
+        RG_TRY(read_scalars_end(hbases.data(), sizeof(int64_t) * hbases.size()));
         std::memcpy(cnt2, hbases.data() + 2 * (npairs + 1), sizeof(cnt2));
         if (cnt2[2] != 0)
             return fail(RG_ERR_HIP, "internal: the emission did not match the "
@@ -2296,16 +2302,17 @@ int build_batched(const rg_pair_desc* pairs, int64_t npairs,
     // ---- CSR by input, transpose -> CSR by output
     DevBuf<int32_t> rp_in, c_in, rp_out, c_out;
     DevBuf<double> v_in, v_out;
-    DevBuf<int64_t> off_out;
     RG_TRY(rp_in.alloc(cell_tot + npairs, s));
     RG_TRY(c_in.alloc(nnz, s));
     RG_TRY(v_in.alloc(nnz, s));
     RG_TRY(rp_out.alloc(out_tot + npairs, s));
     RG_TRY(c_out.alloc(nnz, s));
     RG_TRY(v_out.alloc(nnz, s));
-    RG_TRY(off_out.alloc(out_tot + 1, s));
     rec.reset();
-    RG_TRY(scan_i32_to_i64(hist_out.p, off_out.p, out_tot, s));
+    if (capmode) {
+        RG_TRY(off_out.alloc(out_tot + 1, s));
+        RG_TRY(scan_i32_to_i64(hist_out.p, off_out.p, out_tot, s));
+    }
     k_csr_groups<<<grid_for(n_groups * 64, 256), 256, 0, s>>>(
         d_pcs, pair_of_ing.p, n_groups, cell_off.p, row_off.p, group_mode.p,
         red.p, rp_in.p, c_in.p, v_in.p, off_out.p, c_out.p, v_out.p, nnz);

@@ -82,6 +82,11 @@ struct DevBuf {
 
 // Pinned host scratch for small synchronous device->host reads.
 int read_scalars(const void* dev, void* host, size_t bytes, hipStream_t s);
+// The same read in two steps: _begin queues the copy on s, _end waits for
+// that copy alone (work queued on s in between keeps running) and returns
+// the bytes.  One such read per host thread at a time.
+int read_scalars_begin(const void* dev, size_t bytes, hipStream_t s);
+int read_scalars_end(void* host, size_t bytes);
 
 // Device address, on the current device, of the fixed scratch array of
 // rg_find_cells_2d_noalloc (looked up once per device and kept).

@@ -267,6 +267,44 @@ int read_scalars(const void* dev, void* host, size_t bytes, hipStream_t s)
     return RG_OK;
 }
 
+// A read in two steps: the copy is queued on the stream, and the host waits
+// for that copy alone, so work queued on the stream after the copy runs
+// while the host waits.  One such read per host thread at a time.
+namespace {
+thread_local void* g_split_pinned = nullptr;
+thread_local size_t g_split_size = 0;
+thread_local hipEvent_t g_split_done = nullptr;
+thread_local bool g_split_pending = false;  // begun, not ended (an error path)
+}  // namespace
+
+int read_scalars_begin(const void* dev, size_t bytes, hipStream_t s)
+{
+    // a read begun and never ended may still be copying into the buffer
+    if (g_split_pending) RG_HIP(hipEventSynchronize(g_split_done));
+    g_split_pending = false;
+    if (bytes > g_split_size) {
+        if (g_split_pinned) (void)hipHostFree(g_split_pinned);
+        g_split_pinned = nullptr;
+        g_split_size = 0;
+        const size_t sz = bytes < 4096 ? 4096 : bytes;
+        RG_HIP(hipHostMalloc(&g_split_pinned, sz, hipHostMallocDefault));
+        g_split_size = sz;
+    }
+    if (!g_split_done) RG_HIP(hipEventCreateWithFlags(&g_split_done, hipEventDisableTiming));
+    RG_HIP(hipMemcpyAsync(g_split_pinned, dev, bytes, hipMemcpyDeviceToHost, s));
+    RG_HIP(hipEventRecord(g_split_done, s));
+    g_split_pending = true;
+    return RG_OK;
+}
+
+int read_scalars_end(void* host, size_t bytes)
+{
+    RG_HIP(hipEventSynchronize(g_split_done));
+    g_split_pending = false;
+    std::memcpy(host, g_split_pinned, bytes);
+    return RG_OK;
+}
+
 // ---- scans ---------------------------------------------------------------
 
 namespace {
