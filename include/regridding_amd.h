@@ -197,8 +197,9 @@ int rg_weights_set_shape(rg_weights_t w, int64_t ncx_in, int64_t ncy_in,
 
 /* Launch shape of the apply kernels (process-wide; 0 keeps the current
  * value): tile_w output cells along the last axis per tile row of a
- * 256-cell tile for FP64 values, twice that (up to 256) for FP32 values
- * (default 16; negative: same width, staged input patches off),
+ * 256-cell tile for the FP64 forward apply, twice that (up to 256) for FP32
+ * values and for the adjoint (default 16; negative: same widths, staged
+ * input patches off),
  * images_per_pass images accumulated per pass over the weights (1, 2, 4, 8
  * or 16), lds_kib the staged patch size per block (KiB, 1..64) above which
  * fewer images are staged per pass, and rounds (1: one block per tile loops

@@ -523,10 +523,12 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_apply_jobs(
 
 namespace {
 
-// Tile width for 8-byte values; 4-byte values use tiles twice as wide (up to
-// APPLY_BLOCK), so a staged patch row holds about the same bytes either way.
-// Tuned on the plain 2049^2 -> 1537^2 pair, 16 images: FP64 16 wide, FP32 32
-// wide (0.149 -> 0.129 ms forward, 0.181 -> 0.148 ms adjoint against 16).
+// Tile width of the FP64 forward apply; 4-byte values and the adjoint use
+// tiles twice as wide (up to APPLY_BLOCK).  Tuned on the plain 2049^2 ->
+// 1537^2 pair, 16 images: FP64 forward 16 wide, FP32 32 wide (0.149 ->
+// 0.129 ms forward, 0.181 -> 0.148 ms adjoint against 16); the FP64 adjoint
+// 32 wide (0.252 -> 0.247 ms against 16; its rows are the input cells, and
+// its patches short runs of output rows).
 int g_tile_w = 16;
 int g_images_per_pass = 8;
 // staged patch per block above which fewer images are staged per pass: at
@@ -538,7 +540,7 @@ bool g_use_plan = true;
 bool g_rounds = true;  // one block per tile looping over image groups
 std::mutex g_plan_mu;
 
-ApplyGrid make_grid(const int64_t* shape, int64_t nrows, int64_t value_bytes)
+ApplyGrid make_grid(const int64_t* shape, int64_t nrows, int64_t value_bytes, int dir)
 {
     ApplyGrid G;
     G.gx = shape[0];
@@ -547,7 +549,7 @@ ApplyGrid make_grid(const int64_t* shape, int64_t nrows, int64_t value_bytes)
         G.gx = 1;
         G.gy = nrows;
     }
-    G.tw = value_bytes >= 8 ? g_tile_w : std::min<int64_t>(2 * g_tile_w, APPLY_BLOCK);
+    G.tw = value_bytes >= 8 && dir == 0 ? g_tile_w : std::min<int64_t>(2 * g_tile_w, APPLY_BLOCK);
     if (G.gx == 1 || G.tw > G.gy) G.tw = APPLY_BLOCK;
     G.th = APPLY_BLOCK / G.tw;
     G.ntx = (G.gy + G.tw - 1) / G.tw;
@@ -597,7 +599,7 @@ int launch_apply(rg_weights* W, int dir, const int32_t* rowptr,
                  int64_t src_gy, int64_t batch, hipStream_t s)
 {
     if (nrows == 0 || batch == 0) return RG_OK;
-    const ApplyGrid G = make_grid(shape, nrows, sizeof(T));
+    const ApplyGrid G = make_grid(shape, nrows, sizeof(T), dir);
     rg_weights::Plan* P = nullptr;
     RG_TRY(ensure_plan(W, dir, sizeof(T), rowptr, col, G, src_gy, s, &P));
     const int64_t nb = (G.ntiles + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
