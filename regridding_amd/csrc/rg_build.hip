@@ -1271,7 +1271,10 @@ __global__ __launch_bounds__(HEAVY_WAVES * 64) void k_reduce_heavy(
                     if ((cached ? CB[q] : rcell[src]) == c)
                         red[base + sb + atomicAdd(&C[0], 1)] = rec[src];
                 }
-                if (lane == c) long_list[atomicAdd(n_long, 1)] = (int32_t)i | INT32_MIN;
+                if (lane == c) {  // (k_bucket_long takes the cell's size from nnz_row)
+                    nnz_row[i] = n;
+                    long_list[atomicAdd(n_long, 1)] = (int32_t)i | INT32_MIN;
+                }
                 wave_sync_lds();
                 c++;
                 continue;
@@ -1351,22 +1354,31 @@ __global__ __launch_bounds__(HEAVY_WAVES * 64) void k_reduce_heavy(
                     V[pe[u]] = ve[u];
                 }
             wave_sync_lds();
+            // the run's cells reduced, each written packed after the run's
+            // previous cells (cell_off moves to it): the CSR pass then reads
+            // the run's entries as one dense range instead of one short
+            // range per cell's records
+            uint64_t* Kc = K + (s - sb);
+            double* Vc = V + (s - sb);
+            bool lrc = false;
+            int64_t m = 0;
             if (in) {
-                uint64_t* Kc = K + (s - sb);
-                double* Vc = V + (s - sb);
-                bool lrc = false;
-                int64_t m = bucket_reduce_sorted(Kc, Vc, n, &lrc);
+                m = bucket_reduce_sorted(Kc, Vc, n, &lrc);
+                if (lrc) m = n;  // left sorted, uncompacted, for k_bucket_long
+            }
+            const int64_t dst = base + sb + wave_exclusive(in ? (int32_t)m : 0, lane);
+            if (in) {
                 if (lrc) {
-                    m = n;  // left sorted, uncompacted, for k_bucket_long
                     for (int64_t t = 0; t < m; t++) {
                         KV r;
                         r.key = Kc[t];
                         r.val = Vc[t];
-                        red[base + s + t] = r;
+                        red[dst + t] = r;
                     }
                 } else {
-                    put_reduced_cell(red + base + s, Kc, Vc, m, hist_out);
+                    put_reduced_cell(red + dst, Kc, Vc, m, hist_out);
                 }
+                cell_off[i] = dst;
                 nnz_row[i] = (int32_t)m;
                 if (lrc) long_list[atomicAdd(n_long, 1)] = (int32_t)i;
             }
@@ -1397,7 +1409,9 @@ __global__ void k_bucket_long(const int64_t* __restrict__ cell_off, KV* __restri
          q += gridDim.x * blockDim.x) {
         const int32_t e = long_list[q];
         const int64_t i = e & INT32_MAX;
-        const int64_t start = cell_off[i], n = cell_off[i + 1] - start;
+        // (a long cell keeps all its records: nnz_row holds its record count
+        // until the cell is reduced here)
+        const int64_t start = cell_off[i], n = nnz_row[i];
         KV* r = red + start;
         if (e < 0) {
             const double vol = vol_in[i];
