@@ -45,7 +45,8 @@ typedef struct rg_tuning {
     int32_t block;          /* threads per block of the walk kernels (128; a
                                multiple of 64, at most 512) */
     int32_t flags;          /* RG_TUNE_TIMINGS: record per-phase times;
-                               RG_TUNE_GUESS_OUTSIDE: a test hook */
+                               RG_TUNE_GUESS_OUTSIDE, RG_TUNE_ROW_WAVES: test
+                               hooks */
     int32_t stage_per_edge; /* triples staged per sweep edge by the counting
                                walk (default 8); chunks that emit more are
                                walked a second time */
@@ -56,6 +57,10 @@ typedef struct rg_tuning {
  * the cell-search guess, so the fix-up re-walks most chunks; the result is
  * the same bits, only slower. */
 #define RG_TUNE_GUESS_OUTSIDE 2
+/* Test hook: the row starts take 64 rows per wavefront (one secant search
+ * per lane) whatever the row count, where a build with few rows takes one
+ * row per wavefront; the result is the same bits. */
+#define RG_TUNE_ROW_WAVES 4
 
 /* Counters and (optionally) phase times of a build. */
 typedef struct rg_build_stats {

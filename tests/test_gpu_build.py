@@ -98,6 +98,19 @@ def test_build_fixup_rewalks_bad_guesses(case, K):
     assert h.stats()["n_fixups"] > 0
 
 
+@pytest.mark.parametrize("case", ["c1", "coarse_to_fine", "disjoint_partial",
+                                  "output_covers_input", "very_coarse_to_fine"])
+def test_build_row_starts_many_rows_per_wave(case):
+    """The row starts of 64 rows per wavefront (a test hook; a build with
+    few rows takes one row per wavefront): rows outside the static grid's
+    box, outside its boundary and inside it, each lane running its own
+    row's secant search, give the oracle's bits."""
+    ci, co = _perturbed(*CASES[case]())
+    got = build_2d(ci, co, flags=4).export()
+    for a, b in zip(got, oracle.weights_2d(ci, co)):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("case", ["coarse_to_fine", "mid_groups"])
 def test_build_weights_input(case):
     ci, co = _perturbed(*CASES[case]())
