@@ -841,27 +841,30 @@ RG_HD int64_t bucket_reduce(uint64_t* keys, double* vals, int64_t n,
     return bucket_reduce_sorted(keys, vals, n, long_run);
 }
 
-// Bitonic sorting network over N keys in registers (N a power of two, every
-// index static once unrolled), values following.
+// Sorting network over N keys in registers (N a power of two, every index
+// static once unrolled), values following.
 template <int N, class K>
 RG_HD void sort_network(K* k, double* v)
 {
+    // Batcher's odd-even merge sort (63 comparators for 16 keys where the
+    // bitonic network takes 80, 19 for 8 where it takes 24), unrolled into
+    // registers; the keys come out ascending
 #pragma unroll
-    for (int size = 2; size <= N; size <<= 1) {
+    for (int p = 1; p < N; p <<= 1) {
 #pragma unroll
-        for (int stride = size / 2; stride > 0; stride >>= 1) {
+        for (int q = p; q > 0; q >>= 1) {
 #pragma unroll
-            for (int i = 0; i < N; i++) {
-                const int j = i ^ stride;
-                if (j > i) {
-                    const bool up = (i & size) == 0;
-                    if ((k[i] > k[j]) == up) {
-                        const K tk = k[i];
-                        k[i] = k[j];
-                        k[j] = tk;
-                        const double tv = v[i];
-                        v[i] = v[j];
-                        v[j] = tv;
+            for (int j = q % p; j + q < N; j += 2 * q) {
+#pragma unroll
+                for (int i = 0; i < q; i++) {
+                    const int a = i + j, b = i + j + q;
+                    if (b < N && a / (2 * p) == b / (2 * p) && k[a] > k[b]) {
+                        const K tk = k[a];
+                        k[a] = k[b];
+                        k[b] = tk;
+                        const double tv = v[a];
+                        v[a] = v[b];
+                        v[b] = tv;
                     }
                 }
             }

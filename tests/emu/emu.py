@@ -49,8 +49,20 @@ def lib():
         L.rge_cell_area.argtypes = [_pd, _pd, _i64, _i64, _pd]
         L.rge_index_secant.restype = _i64
         L.rge_index_secant.argtypes = [_pd, _pd, _i64, _i64, ctypes.c_double, ctypes.c_double]
+        L.rge_sort_network.restype = None
+        L.rge_sort_network.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _i64, ctypes.c_int]
         _lib = L
     return _lib
+
+
+def sort_network(keys, n):
+    """The kernels' register sorting network over each row of `keys`
+    (rows of n = 8 or 16 uint64 keys): the sorted keys and, as values, the
+    positions they came from."""
+    k = np.ascontiguousarray(keys, dtype=np.uint64).reshape(-1, n).copy()
+    v = np.tile(np.arange(n, dtype=np.float64), (k.shape[0], 1))
+    lib().rge_sort_network(k.ctypes.data, v.ctypes.data, k.shape[0], int(n))
+    return k, v.astype(np.int64)
 
 
 def _d(a):

@@ -425,6 +425,18 @@ int64_t rge_build(const double* xi, const double* yi, int64_t nxi, int64_t nyi,
 
 void rge_free(void* p) { free(p); }
 
+// sort_network<N> (N = 8 or 16) on each of `count` rows of N keys, with the
+// keys' positions as values.
+void rge_sort_network(uint64_t* keys, double* vals, int64_t count, int n)
+{
+    for (int64_t r = 0; r < count; r++) {
+        if (n == 8)
+            sort_network<8, uint64_t>(keys + 8 * r, vals + 8 * r);
+        else
+            sort_network<16, uint64_t>(keys + 16 * r, vals + 16 * r);
+    }
+}
+
 double rge_reduceat_segment(const double* a, int64_t n)
 {
     return reduceat_segment(a, n);
