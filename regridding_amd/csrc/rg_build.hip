@@ -850,13 +850,18 @@ __global__ __launch_bounds__(WALK_MAX_BLOCK) void k_scatter(const Build* __restr
         }
         wave_sync_lds();
         // claims, and each entry's offset in the round's staging (a wave
-        // prefix over the table, SCAT_HT / 64 entries per lane)
+        // prefix over the table, SCAT_HT / 64 entries per lane); the claims'
+        // results are kept in registers and written to the table only after
+        // the round's records are laid out, so their latency overlaps the
+        // layout
         int32_t c4[SCAT_HT / 64], sum = 0;
+        uint32_t b4[SCAT_HT / 64];
 #pragma unroll
         for (int u = 0; u < SCAT_HT / 64; u++) {
             const int h = lane * (SCAT_HT / 64) + u;
             c4[u] = H.key[h] >= 0 ? H.cnt[h] : 0;
-            if (c4[u] > 0) H.base[h] = atomicAdd(&cursor[H.key[h]], (uint32_t)c4[u]);
+            b4[u] = 0;
+            if (c4[u] > 0) b4[u] = atomicAdd(&cursor[H.key[h]], (uint32_t)c4[u]);
             sum += c4[u];
         }
         int32_t o = wave_exclusive(sum, lane);
@@ -886,6 +891,9 @@ __global__ __launch_bounds__(WALK_MAX_BLOCK) void k_scatter(const Build* __restr
                     atomicAdd(overflow, 1);
             }
         }
+#pragma unroll
+        for (int u = 0; u < SCAT_HT / 64; u++)
+            if (c4[u] > 0) H.base[lane * (SCAT_HT / 64) + u] = b4[u];
         wave_sync_lds();
         for (int32_t i = lane; i < staged; i += 64) {
             const int h = H.rslot[i];
