@@ -155,16 +155,16 @@ def pack_pairs(pairs) -> PackedPairs:
 
 def build_2d_batched(pairs, edges_per_lane: int = 0, block: int = 0,
                      timings: bool = False, stage_per_edge: int = 0,
-                     stream: int = 0) -> list[Weights]:
+                     stream: int = 0, flags: int = 0) -> list[Weights]:
     """Build many independent grid pairs together (one launch per phase for
     the whole batch).  ``pairs`` is a PackedPairs or a sequence accepted by
     pack_pairs.  Returns one handle per pair, each bit-identical to the
-    pair's own build_2d."""
+    pair's own build_2d.  ``flags`` adds ``RG_TUNE_*`` bits (test hooks)."""
     packed = pairs if isinstance(pairs, PackedPairs) else PackedPairs(pairs)
     if packed.n == 0:
         return []
-    t = rg_tuning(int(edges_per_lane), int(block), RG_TUNE_TIMINGS if timings else 0,
-                  int(stage_per_edge))
+    t = rg_tuning(int(edges_per_lane), int(block),
+                  (RG_TUNE_TIMINGS if timings else 0) | int(flags), int(stage_per_edge))
     hs = (ctypes.c_void_p * packed.n)()
     check(lib().rg_build_conservative_2d_batched(packed.desc, packed.n, ctypes.byref(t), hs,
                                                  stream))

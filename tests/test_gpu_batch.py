@@ -61,6 +61,17 @@ def test_batch_matches_single_and_oracle():
         assert h.stats()["n_chunks"] == single.stats()["n_chunks"]
 
 
+def test_batch_row_starts_many_rows_per_wave():
+    """The row starts with 64 rows per wavefront (a test hook: a batch this
+    small takes one row per wavefront), so a wavefront's rows span passes
+    and pairs: every pair still matches its own single-pair build."""
+    pairs = _mixed_pairs()
+    hs = build_2d_batched(pairs, flags=4)
+    for (ci, co), h in zip(pairs, hs):
+        for a, b in zip(h.export(), build_2d(ci, co).export()):
+            assert np.array_equal(a, b)
+
+
 def test_batch_apply_and_adjoint():
     pairs = _mixed_pairs()
     hs = build_2d_batched(pairs)
