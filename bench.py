@@ -272,6 +272,11 @@ def main() -> int:
             m = measured.get(r["kernel"].split(" ")[0])
             if m:
                 r["traffic"] = m["read_bytes"] + m["write_bytes"]
+                # the rate the kernel moves its measured bytes at (for the
+                # walk, whose algorithmic unit is FP64 work, this is what
+                # bounds it more nearly than its arithmetic)
+                r["traffic_GBps"] = round(r["traffic"] / (r["ms_per_launch"] * 1e-3) / 1e9, 1)
+                r["traffic_frac_hbm"] = round(r["traffic_GBps"] / HBM_PEAK_GBPS, 4)
     rooflines.sort(key=lambda r: -r["ms_per_launch"])
     log("rooflines: " + "; ".join(f"{r['kernel'].split(' ')[0]} {r['ms_per_launch']:.3f} ms "
                                    f"{r['achieved']} {r['unit']} ({100 * r['frac']:.1f}%)" for r in rooflines))
