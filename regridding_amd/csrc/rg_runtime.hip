@@ -274,6 +274,7 @@ namespace {
 thread_local void* g_split_pinned = nullptr;
 thread_local size_t g_split_size = 0;
 thread_local hipEvent_t g_split_done = nullptr;
+thread_local int g_split_device = -1;       // the device g_split_done belongs to
 thread_local bool g_split_pending = false;  // begun, not ended (an error path)
 }  // namespace
 
@@ -290,7 +291,16 @@ int read_scalars_begin(const void* dev, size_t bytes, hipStream_t s)
         RG_HIP(hipHostMalloc(&g_split_pinned, sz, hipHostMallocDefault));
         g_split_size = sz;
     }
-    if (!g_split_done) RG_HIP(hipEventCreateWithFlags(&g_split_done, hipEventDisableTiming));
+    int device = 0;
+    RG_HIP(hipGetDevice(&device));
+    if (g_split_done && g_split_device != device) {  // an event records on its own device only
+        (void)hipEventDestroy(g_split_done);
+        g_split_done = nullptr;
+    }
+    if (!g_split_done) {
+        RG_HIP(hipEventCreateWithFlags(&g_split_done, hipEventDisableTiming));
+        g_split_device = device;
+    }
     RG_HIP(hipMemcpyAsync(g_split_pinned, dev, bytes, hipMemcpyDeviceToHost, s));
     RG_HIP(hipEventRecord(g_split_done, s));
     g_split_pending = true;
