@@ -716,6 +716,7 @@ struct JobUpload {
     void* buf = nullptr;
     size_t cap = 0;
     hipEvent_t done = nullptr;
+    int device = -1;  // the device `done` belongs to
     bool recorded = false;
 };
 
@@ -773,7 +774,16 @@ int launch_jobs_common(const rg_apply_job* jobs, int64_t njobs, bool f32,
         RG_HIP(hipHostMalloc(&up.buf, sz, hipHostMallocDefault));
         up.cap = sz;
     }
-    if (!up.done) RG_HIP(hipEventCreateWithFlags(&up.done, hipEventDisableTiming));
+    int device = 0;
+    RG_HIP(hipGetDevice(&device));
+    if (up.done && up.device != device) {  // an event records on its own device only
+        (void)hipEventDestroy(up.done);
+        up.done = nullptr;
+    }
+    if (!up.done) {
+        RG_HIP(hipEventCreateWithFlags(&up.done, hipEventDisableTiming));
+        up.device = device;
+    }
     std::memcpy(up.buf, table.data(), tbytes);
     std::memcpy((char*)up.buf + poff, prefix.data(), pbytes);
     DevBuf<unsigned char> dt;
