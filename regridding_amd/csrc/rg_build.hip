@@ -944,6 +944,9 @@ constexpr int RED_WAVES = 1;
 constexpr int HEAVY_WAVES = 2;  // k_reduce_heavy's
 constexpr int RED_CAP = 1024;    // LDS entries per wavefront (16 KiB)
 constexpr int RED_NET = 16;      // largest cell sorted in registers
+// the lane-strided layout of a group whose cells all fit RED_NET entries
+// puts entry j of cell c at j * 64 + c: RED_NET rows of 64 must fit
+static_assert(RED_CAP >= RED_NET * 64, "the lane-strided layout needs RED_NET * 64 entries");
 
 // One cell's terms: divided by the cell area (and weighted), sorted by key
 // in registers through a network of N, written back in place.
